@@ -1,0 +1,254 @@
+/*
+ * rfmf.h -- C ABI of the MI355X-native RFMF annealer (librfmf.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns an rfmf_status;
+ * rfmf_last_error() gives the calling thread's last detail message.  Device
+ * memory is owned by the library: the one-shot entry points allocate and free
+ * inside the call, as the reference's annealGpu does; contexts keep graphs
+ * and chain state resident in HBM between calls.
+ *
+ * Reference interfaces are cited as path:line in the reference checkout.
+ */
+#ifndef RFMF_H
+#define RFMF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rfmf_status {
+    RFMF_OK = 0,
+    RFMF_ERR_INVALID_ARG = 1,   /* sizes, null pointers, t0 <= 0, P out of range */
+    RFMF_ERR_UNSUPPORTED = 2,   /* int64 score path, |S| bound >= 2^30 */
+    RFMF_ERR_HIP = 3,           /* HIP runtime error (message has the detail) */
+    RFMF_ERR_NO_DEVICE = 4,
+    RFMF_ERR_OUT_OF_MEMORY = 5,
+    RFMF_ERR_BETA_CAP = 6,      /* more than 33,000,000 beta runs */
+    RFMF_ERR_STEP_CAP = 7,      /* more than 10^12 sweeps per chain */
+    RFMF_ERR_IO = 8,            /* unreadable or malformed instance file */
+    RFMF_ERR_NOT_FOUND = 9,     /* unknown graph / request id */
+    RFMF_ERR_ADMISSION = 10,    /* request larger than the server's HBM budget */
+    RFMF_ERR_STATE = 11         /* call not valid in the object's state */
+} rfmf_status;
+
+const char* rfmf_status_string(rfmf_status s);
+const char* rfmf_last_error(void);
+/* Library build identifier ("rfmf <version> gfx950"). */
+const char* rfmf_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Graph views and host helpers                                            */
+/* ---------------------------------------------------------------------- */
+
+/* Dense CSR view: rows sorted by neighbour id, each undirected edge stored
+ * twice.  Replaces anneallib::AnnealCsr as seen by the kernel
+ * (src/immutable_graph.hpp:154-157, src/anneal_gpu.cuh:23-28). */
+typedef struct rfmf_csr {
+    const uint32_t* offsets;      /* n + 1 */
+    const int32_t* edge_to;       /* m2 */
+    const int32_t* edge_weight;   /* m2 */
+    int64_t n;
+    int64_t m2;
+} rfmf_csr;
+
+/* Loader statistics, anneallib::ReadStats (src/read_gset.hpp:11-18). */
+typedef struct rfmf_read_stats {
+    int64_t num_vertices;
+    int64_t num_edges;
+    int64_t zero_dropped;
+    int32_t float_weights;
+    int32_t pad;
+    double scale;
+    int64_t max_abs_weighted_degree;
+} rfmf_read_stats;
+
+/* Owning host CSR (the frozen AnnealCsr built by the load path). */
+typedef struct rfmf_graph rfmf_graph;
+
+/* read_instance_from_file + removeDegreeZeroVertices + AnnealCsr
+ * (src/read_gset.hpp:32-34, src/host_common.cpp:316-327,463-475). */
+rfmf_status rfmf_graph_read_gset(const char* path, double forced_scale, rfmf_graph** out,
+                                 rfmf_read_stats* stats);
+/* Graph::addEdge over an edge list in order (first weight wins, zero weights
+ * kept as given), then AnnealCsr (src/graph.cpp:11-17). */
+rfmf_status rfmf_graph_from_edges(const int64_t* u, const int64_t* v, const int64_t* w,
+                                  int64_t m, rfmf_graph** out);
+void rfmf_graph_free(rfmf_graph* g);
+/* View valid while the graph lives. */
+rfmf_status rfmf_graph_view(const rfmf_graph* g, rfmf_csr* view);
+/* Dense id -> original id (n entries). */
+const int64_t* rfmf_graph_original_ids(const rfmf_graph* g);
+
+/* anneallib::solver::cutValueCsr (src/max_cut.hpp:16). */
+int64_t rfmf_cut_value(const rfmf_csr* g, const int8_t* spin);
+/* maxWeightedDegree (src/host_common.h:111-112). */
+int64_t rfmf_max_weighted_degree(const rfmf_csr* g);
+/* annealScoreSafeInt32 (src/host_common.h:106-109): 1 if the packed int32
+ * score path is safe, else 0. */
+int32_t rfmf_score_safe_int32(int64_t total_steps, double t0, int64_t delta_bound);
+/* betaFixedForStep (src/host_common.h:73-76). */
+int32_t rfmf_beta_fixed_for_step(uint64_t step, double t0);
+/* annealTotalSteps (src/host_common.h:67-68). */
+rfmf_status rfmf_total_steps(const uint64_t* iters, int64_t n, int64_t* total);
+/* Rng(raw seed) + generateRandomSpins (src/rng.hpp:11-58,
+ * src/host_common.cpp:171-178). */
+rfmf_status rfmf_initial_spins(uint64_t raw_seed, int64_t n, int8_t* out);
+
+/* Run-length beta tables: run r of a table covers 0-based steps
+ * [run_end[r-1], run_end[r]) with Q20 value run_val[r]; chain c uses the table
+ * starting at chain_run_off[c] (src/host_common.h:82-94). */
+typedef struct rfmf_beta_tables {
+    const int64_t* run_end;
+    const int32_t* run_val;
+    const int64_t* chain_run_off;
+    int64_t n_runs;
+} rfmf_beta_tables;
+
+/* buildBetaRunTables (src/host_common.h:101-104).  Call with run_end/run_val
+ * NULL to get the run count only.  chain_run_off receives `chains` entries. */
+rfmf_status rfmf_beta_tables_log(const double* t0s, int64_t chains, int64_t total_steps,
+                                 int64_t* run_end, int32_t* run_val, int64_t cap,
+                                 int64_t* n_runs, int64_t* chain_run_off, int64_t* n_tables);
+/* Exact-integer linear schedule beta(s) = b0 + ((b1-b0)*(s-1))/(total-1), Q20,
+ * one table shared by all chains (no reference counterpart; the reference's
+ * only schedule is the log one). */
+rfmf_status rfmf_beta_tables_linear(int32_t b0, int32_t b1, int64_t total_steps,
+                                    int64_t* run_end, int32_t* run_val, int64_t cap,
+                                    int64_t* n_runs);
+
+/* ---------------------------------------------------------------------- */
+/* One-shot annealing: drop-in for gpu3::annealGpu                         */
+/* ---------------------------------------------------------------------- */
+
+/* gpu3::CorrMode (src/anneal_gpu.cuh:42-45).  Both are output-invariant; the
+ * HIP kernels use one corr formulation for both. */
+typedef enum rfmf_corr_mode { RFMF_CORR_WALK = 0, RFMF_CORR_WARPWALK = 1 } rfmf_corr_mode;
+
+/* gpu3::LaunchConfig (src/anneal_gpu.cuh:52-58). */
+typedef struct rfmf_launch_cfg {
+    int32_t corr;      /* rfmf_corr_mode */
+    int32_t device;    /* HIP device ordinal */
+    int32_t s_width;   /* 32 or 64: accepted; the state word width is chosen from B */
+    int32_t reserved;
+} rfmf_launch_cfg;
+
+/* gpu3::annealGpu (src/anneal_gpu.cuh:76-88): C chains from initial_spins
+ * (C x n, +/-1) with one t0 and one solver seed each, phases/iters shared.
+ * Outputs end_spins / best_spins (C x n), the kernel-only time, and the
+ * incrementally tracked best / final cuts (may be NULL).  n == 0 or
+ * n_phases == 0 returns the initial state without device work, as the
+ * reference does (src/anneal_kernels.cu:856-868). */
+rfmf_status rfmf_anneal(const rfmf_csr* g, int64_t max_weighted_deg, const int8_t* initial_spins,
+                        int64_t chains, const double* t0s, const uint64_t* phases,
+                        const uint64_t* iters, int64_t n_phases, const uint64_t* solver_seeds,
+                        const rfmf_launch_cfg* cfg, int8_t* end_spins, int8_t* best_spins,
+                        float* kernel_ms, int64_t* kernel_best_vals, int64_t* kernel_final_vals);
+
+/* rfmf_anneal with caller-built beta tables (any monotone or arbitrary Q20
+ * schedule) instead of t0s.  Each table must cover sum(iters) steps. */
+rfmf_status rfmf_anneal_tables(const rfmf_csr* g, int64_t max_weighted_deg,
+                               const int8_t* initial_spins, int64_t chains,
+                               const rfmf_beta_tables* beta, const uint64_t* phases,
+                               const uint64_t* iters, int64_t n_phases,
+                               const uint64_t* solver_seeds, const rfmf_launch_cfg* cfg,
+                               int8_t* end_spins, int8_t* best_spins, float* kernel_ms,
+                               int64_t* kernel_best_vals, int64_t* kernel_final_vals);
+
+/* ---------------------------------------------------------------------- */
+/* Resident device context: graphs in HBM, multi-graph batches, serving.   */
+/* No reference counterpart (the reference allocates per call and runs one */
+/* graph per launch); these serve callers that batch many graphs or serve  */
+/* a request stream.                                                       */
+/* ---------------------------------------------------------------------- */
+
+typedef struct rfmf_ctx rfmf_ctx;
+
+rfmf_status rfmf_ctx_create(int32_t device, rfmf_ctx** out);
+void rfmf_ctx_destroy(rfmf_ctx* ctx);
+/* Copies the CSR into HBM; returns a graph id valid until released. */
+rfmf_status rfmf_ctx_upload_graph(rfmf_ctx* ctx, const rfmf_csr* g, int32_t* graph_id);
+rfmf_status rfmf_ctx_release_graph(rfmf_ctx* ctx, int32_t graph_id);
+/* Device properties seen by the context. */
+rfmf_status rfmf_ctx_device_info(rfmf_ctx* ctx, int32_t* cu_count, int64_t* lds_per_block,
+                                 int64_t* hbm_bytes, char* name, int32_t name_cap);
+
+/* A job: `chains` chains of one resident graph with their own schedule. */
+typedef struct rfmf_job {
+    int32_t graph_id;
+    int32_t pad;
+    int64_t chains;
+    const int8_t* initial_spins;   /* chains x n */
+    const uint64_t* solver_seeds;  /* chains */
+    rfmf_beta_tables beta;         /* chain_run_off has `chains` entries */
+    const uint64_t* phases;        /* P per phase, shared by the job's chains */
+    const uint64_t* iters;
+    int64_t n_phases;
+    int8_t* end_spins;             /* out, chains x n (may be NULL) */
+    int8_t* best_spins;            /* out, chains x n (may be NULL) */
+    int64_t* best_vals;            /* out, chains (may be NULL) */
+    int64_t* final_vals;           /* out, chains (may be NULL) */
+} rfmf_job;
+
+/* Launch knobs; every one is output-invariant.  Zero fields mean "auto"
+ * (the autotuner's cached choice, else the built-in default). */
+typedef struct rfmf_tuning {
+    int64_t slice_sweeps;    /* sweeps per launch (0: whole run in one launch) */
+    int32_t grid_cap;        /* max resident workgroups (0: occupancy x CUs) */
+    int32_t use_hip_graph;   /* 1: replay the slice launches from a captured hipGraph */
+    int32_t xcd_affinity;    /* 1: keep each graph's chains on one XCD where possible */
+    int32_t reserved[3];
+} rfmf_tuning;
+
+/* Runs every job's chains to completion in persistent launches that pack all
+ * jobs together; returns the kernel-only time of all launches. */
+rfmf_status rfmf_batch_run(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs,
+                           const rfmf_tuning* tuning, float* kernel_ms);
+
+/* HBM bytes a job of this shape keeps resident (CSR + chain state + tables). */
+int64_t rfmf_footprint_bytes(int64_t n, int64_t m2, int64_t chains, int64_t beta_runs,
+                             int64_t n_phases);
+
+/* ---------------------------------------------------------------------- */
+/* Serving: continuous batching of MAX-CUT solve requests.                 */
+/* ---------------------------------------------------------------------- */
+
+typedef struct rfmf_server rfmf_server;
+
+typedef struct rfmf_server_stats {
+    int64_t submitted;
+    int64_t completed;
+    int64_t rejected;
+    int64_t active_chains;
+    int64_t resident_bytes;
+    int64_t slices;
+    double chain_steps;        /* sweeps executed, all chains */
+    double busy_kernel_ms;     /* kernel-only time of all slices */
+} rfmf_server_stats;
+
+/* A server owns a worker thread that runs slices of `slice_sweeps` sweeps
+ * over every admitted chain and admits queued requests between slices while
+ * their HBM footprint fits `hbm_budget_bytes`. */
+rfmf_status rfmf_server_create(rfmf_ctx* ctx, int64_t hbm_budget_bytes, int64_t slice_sweeps,
+                               rfmf_server** out);
+/* Queues a request (graph copied, inputs copied); outputs are written into
+ * the job's output pointers when the request completes (they must stay valid
+ * until then).  job->graph_id is ignored. */
+rfmf_status rfmf_server_submit(rfmf_server* srv, const rfmf_csr* g, const rfmf_job* job,
+                               int64_t* request_id);
+/* done = 1 when the request's outputs are written.  Times are seconds since
+ * the server's creation (submit, admission, completion). */
+rfmf_status rfmf_server_poll(rfmf_server* srv, int64_t request_id, int32_t* done,
+                             double* t_submit, double* t_admit, double* t_done);
+rfmf_status rfmf_server_wait(rfmf_server* srv, int64_t request_id, double timeout_s);
+rfmf_status rfmf_server_stats_get(rfmf_server* srv, rfmf_server_stats* out);
+/* Drains the queue (finishes every admitted and queued request), stops the
+ * worker and frees device state. */
+void rfmf_server_destroy(rfmf_server* srv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RFMF_H */

@@ -1,0 +1,559 @@
+// anneal_kernels.hip -- CDNA4 (gfx950) kernels of the multi-chain
+// rejection-free multiple-flip annealer.
+//
+// Semantics follow the reference kernel mcKernel
+// (src/anneal_kernels.cu:208-491): per sweep, SELECT scores every vertex from
+// the pre-sweep state (score = Gumbel + beta * spin*S, Philox4x32-10 keyed by
+// the chain's solver seed, counter (sweep, v>>3)) and keeps the per-group
+// argmax of group g = v mod P (ties to the smaller id); FLIP flips every
+// group's winner, adds the winner-winner coupling corr[] so the cut advances
+// exactly as sequential group-order flips would, and scatters the neighbour
+// field updates.
+//
+// The execution is MI355X-native and differs from the reference throughout:
+//   * one 256-thread workgroup (4 wave64) anneals one chain at a time, and the
+//     chain's whole state lives in LDS for the length of a slice: one packed
+//     word per vertex (spin bit | biased S, 8/16/32 bits chosen from the
+//     graph's max weighted degree) plus a best-spin bitmask;
+//   * the per-group argmax is kept in registers (each lane's 8 vertex slots
+//     map to fixed groups when P divides 8*threads), folded across lanes with
+//     a butterfly and across the 4 waves through LDS: no atomics on the hot
+//     path;
+//   * the winner map for corr[] is implicit: vertex x can only win group
+//     x mod P, so "x is an earlier winner" is pick[x mod P] == x with
+//     x mod P < g, and no n-sized winnerGroup array exists;
+//   * neighbour-field updates are 32-bit LDS atomics on the packed words;
+//   * the kernel is persistent over a work list of (graph, chain) items and
+//     runs each item for a slice of sweeps, saving the state to HBM at the end,
+//     so the same kernel serves single-graph runs, multi-graph batches and the
+//     serving loop.
+#include <hip/hip_runtime.h>
+
+#include "anneal_device.hpp"
+#include "anneal_launch.hpp"
+
+namespace rfmf {
+namespace dev {
+
+// Q20 Gumbel quantile table coarse[i] = round(-ln(-ln((2i+1)/32)) * 2^20);
+// the values must equal the reference's (src/philox.cuh:31-34) for the
+// Gumbel stream to be bit-identical.
+__constant__ int32_t kGumbelCoarse[16] = {
+    -1303301, -903532, -648633, -438929, -249398, -68827,  109563,  290966,
+    480289,   683080,  906615,  1161749, 1466888, 1858654, 2430921, 3617486};
+
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+// Philox4x32-10 (Salmon et al. 2011): 10 rounds, key bumped between rounds.
+// The step words and key are wave-uniform, so the compiler keeps the first
+// round's uniform products on the scalar unit.
+__device__ __forceinline__ void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2,
+                                         uint32_t& c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = static_cast<uint64_t>(kPhiloxM0) * c0;
+        const uint64_t p1 = static_cast<uint64_t>(kPhiloxM1) * c2;
+        const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = static_cast<uint32_t>(p0 >> 32) ^ c3 ^ k1;
+        c1 = static_cast<uint32_t>(p1);
+        c3 = static_cast<uint32_t>(p0);
+        c0 = n0;
+        c2 = n2;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+}
+
+// 8 consecutive words of a vertex block, as loaded from LDS.
+template <int kBits> struct Block8 {
+    static constexpr int kDw = kBits / 4;   // dwords holding 8 words
+    uint32_t d[kDw];
+    __device__ __forceinline__ void load(const uint32_t* wl, int b) {
+        if constexpr (kBits == 8) {
+            const uint2 q = reinterpret_cast<const uint2*>(wl)[b];
+            d[0] = q.x; d[1] = q.y;
+        } else if constexpr (kBits == 16) {
+            const uint4 q = reinterpret_cast<const uint4*>(wl)[b];
+            d[0] = q.x; d[1] = q.y; d[2] = q.z; d[3] = q.w;
+        } else {
+            const uint4 q0 = reinterpret_cast<const uint4*>(wl)[2 * b];
+            const uint4 q1 = reinterpret_cast<const uint4*>(wl)[2 * b + 1];
+            d[0] = q0.x; d[1] = q0.y; d[2] = q0.z; d[3] = q0.w;
+            d[4] = q1.x; d[5] = q1.y; d[6] = q1.z; d[7] = q1.w;
+        }
+    }
+    __device__ __forceinline__ uint32_t word(int j) const {
+        if constexpr (kBits == 32) {
+            return d[j];
+        } else {
+            constexpr int per = 32 / kBits;
+            return (d[j / per] >> (kBits * (j % per))) & ((1u << kBits) - 1u);
+        }
+    }
+    // Spin bits of the 8 words as one byte (bit j = vertex 8b+j is +1).
+    __device__ __forceinline__ uint32_t spinByte() const {
+        uint32_t m = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) m |= ((word(j) >> (kBits - 1)) & 1u) << j;
+        return m;
+    }
+};
+
+template <int kBits>
+__device__ __forceinline__ uint32_t ldsWord(const uint32_t* wl, int x) {
+    if constexpr (kBits == 32) {
+        return wl[x];
+    } else {
+        constexpr int per = 32 / kBits;
+        return (wl[x / per] >> (kBits * (x % per))) & ((1u << kBits) - 1u);
+    }
+}
+
+// delta(x) = spin(x) * S(x) from a packed word.
+template <int kBits>
+__device__ __forceinline__ int32_t wordDelta(uint32_t w) {
+    using F = WordFmt<kBits>;
+    const int32_t S = static_cast<int32_t>(w & F::kField) - F::kBias;
+    return (w & F::kSpin) ? S : -S;
+}
+
+template <int kBits>
+__device__ __forceinline__ void ldsAddField(uint32_t* wl, int x, int32_t d) {
+    constexpr int per = 32 / kBits;
+    const int sh = kBits * (x % per);
+    atomicAdd(&wl[x / per], static_cast<uint32_t>(d) << sh);
+}
+
+template <int kBits>
+__device__ __forceinline__ void ldsFlipSpin(uint32_t* wl, int x) {
+    constexpr int per = 32 / kBits;
+    const int sh = kBits * (x % per);
+    atomicXor(&wl[x / per], WordFmt<kBits>::kSpin << sh);
+}
+
+__device__ __forceinline__ uint64_t packKey(int32_t score, int32_t v) {
+    if (v < 0) return 0ull;
+    return (static_cast<uint64_t>(static_cast<uint32_t>(score) ^ 0x80000000u) << 32) |
+           static_cast<uint64_t>(~static_cast<uint32_t>(v));
+}
+
+// (score, id) argmax with ties to the smaller id; an empty slot has v = -1
+// and score INT32_MIN, which no real score reaches under the int32 gate.
+__device__ __forceinline__ void keepBetter(int32_t& s, int32_t& v, int32_t s2, int32_t v2) {
+    const bool take = (s2 > s) || (s2 == s && static_cast<uint32_t>(v2) < static_cast<uint32_t>(v));
+    s = take ? s2 : s;
+    v = take ? v2 : v;
+}
+
+__device__ __forceinline__ int64_t waveSum64(int64_t x) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const int32_t lo = __shfl_xor(static_cast<int32_t>(x), m, 64);
+        const int32_t hi = __shfl_xor(static_cast<int32_t>(x >> 32), m, 64);
+        x += (static_cast<int64_t>(hi) << 32) | static_cast<uint32_t>(lo);
+    }
+    return x;
+}
+
+template <int kBits, bool kMul24, int NT>
+__global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ items,
+                                                  int nItems, uint32_t* __restrict__ sched,
+                                                  LdsPlan plan, uint64_t sliceSweeps) {
+    constexpr int NW = NT / 64;
+    using F = WordFmt<kBits>;
+    extern __shared__ __align__(16) unsigned char lds[];
+    uint32_t* wl = reinterpret_cast<uint32_t*>(lds + plan.wordsOff);
+    uint32_t* bl = reinterpret_cast<uint32_t*>(lds + plan.bestOff);
+    uint8_t* bl8 = reinterpret_cast<uint8_t*>(lds + plan.bestOff);
+    uint64_t* part = reinterpret_cast<uint64_t*>(lds + plan.partOff);
+    int32_t* pk = reinterpret_cast<int32_t*>(lds + plan.pickOff);
+    int32_t* ps = reinterpret_cast<int32_t*>(lds + plan.spinOff);
+    int32_t* pd = reinterpret_cast<int32_t*>(lds + plan.deltaOff);
+    int32_t* lut = reinterpret_cast<int32_t*>(lds + plan.lutOff);
+    int64_t* dpart = reinterpret_cast<int64_t*>(lds + plan.dpartOff);
+    int32_t* misc = reinterpret_cast<int32_t*>(lds + plan.miscOff);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int pstride = plan.partStride;
+
+    if (tid < 16) lut[tid] = kGumbelCoarse[tid];
+
+    for (;;) {
+        if (tid == 0) misc[0] = static_cast<int32_t>(atomicAdd(&sched[0], 1u));
+        __syncthreads();
+        const int itemIdx = __builtin_amdgcn_readfirstlane(misc[0]);
+        if (itemIdx >= nItems) break;
+
+        const ChainItem& it = items[itemIdx];
+        const int n = it.n;
+        const uint32_t* __restrict__ offsets = it.offsets;
+        const int2* __restrict__ edges = reinterpret_cast<const int2*>(it.edges);
+        ChainState* st = it.st;
+        const int64_t* betaEnd = it.betaEnd;
+        const int32_t* betaVal = it.betaVal;
+        const uint64_t* phaseEnd = it.phaseEnd;
+        const uint32_t* phaseP = it.phaseP;
+        const int nRuns = it.nRuns;
+        const int nPhases = it.nPhases;
+        const uint32_t k0 = static_cast<uint32_t>(it.key);
+        const uint32_t k1 = static_cast<uint32_t>(it.key >> 32);
+        // ---- load the chain's state into LDS ----
+        const int wq = static_cast<int>(wordBytes(n, kBits) / 16);
+        const int bq = static_cast<int>(bestBytes(n) / 16);
+        {
+            const uint4* gw = reinterpret_cast<const uint4*>(it.words);
+            uint4* lw = reinterpret_cast<uint4*>(wl);
+            for (int i = tid; i < wq; i += NT) lw[i] = gw[i];
+            const uint4* gb = reinterpret_cast<const uint4*>(it.bestbits);
+            uint4* lb = reinterpret_cast<uint4*>(bl);
+            for (int i = tid; i < bq; i += NT) lb[i] = gb[i];
+        }
+        int64_t cur = st->cur;
+        int64_t best = st->best;
+        uint64_t step = st->step;
+        int improved = st->improved;
+        const uint64_t remaining = it.stepEnd > step ? it.stepEnd - step : 0;
+        const uint64_t stepEnd = step + (remaining < sliceSweeps ? remaining : sliceSweeps);
+        for (int g = tid; g < plan.slotWords; g += NT) part[g] = 0ull;
+        __syncthreads();
+
+        // Phase and beta-run cursors for the slice's first sweep.
+        int ph = 0;
+        while (ph < nPhases - 1 && phaseEnd[ph] <= step) ++ph;
+        int run = 0;
+        {
+            int lo = 0, hi = nRuns - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (static_cast<uint64_t>(betaEnd[mid]) > step) hi = mid; else lo = mid + 1;
+            }
+            run = lo;
+        }
+
+        const int nFull = n >> 3;
+        const int nBlk = (n + 7) >> 3;
+
+        while (step < stepEnd) {
+            while (ph < nPhases - 1 && phaseEnd[ph] <= step) ++ph;
+            while (run < nRuns - 1 && static_cast<uint64_t>(betaEnd[run]) <= step) ++run;
+            const int P = static_cast<int>(phaseP[ph]);
+            const int32_t beta = betaVal[run];
+            const uint64_t s1 = step + 1;   // 1-based sweep id, the Philox counter
+            const uint32_t sLo = static_cast<uint32_t>(s1);
+            const uint32_t sHi = static_cast<uint32_t>(s1 >> 32);
+            const bool pow2 = (P & (P - 1)) == 0;
+            const bool laneReduce = pow2 && P <= kMaxLaneReduceP && ((8 * NT) % P) == 0;
+
+            // ---------------- SELECT ----------------
+            int32_t accS[8], accV[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) { accS[j] = INT32_MIN; accV[j] = -1; }
+
+            auto scoreBlock = [&](int b, bool tail) {
+                Block8<kBits> blk;
+                blk.load(wl, b);
+                if (improved) bl8[b] = static_cast<uint8_t>(blk.spinByte());
+                uint32_t c0 = sLo, c1 = sHi, c2 = static_cast<uint32_t>(b), c3 = 0u;
+                philox10(c0, c1, c2, c3, k0, k1);
+                const uint32_t o[4] = {c0, c1, c2, c3};
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int v = 8 * b + j;
+                    if (tail && v >= n) continue;
+                    const uint32_t field = (j & 1) ? (o[j >> 1] >> 16) : o[j >> 1];
+                    const uint32_t r = field & 0x7FFu;
+                    const int32_t gum = lut[r & 15u] + static_cast<int32_t>(r >> 4);
+                    const int32_t delta = wordDelta<kBits>(blk.word(j));
+                    const int32_t bd = kMul24 ? __mul24(beta, delta) : beta * delta;
+                    const int32_t score = gum + bd;
+                    if (laneReduce) {
+                        if (score > accS[j]) { accS[j] = score; accV[j] = v; }
+                    } else {
+                        const int g = pow2 ? (v & (P - 1)) : (v % P);
+                        atomicMax(reinterpret_cast<unsigned long long*>(&part[g]),
+                                  static_cast<unsigned long long>(packKey(score, v)));
+                    }
+                }
+            };
+            for (int b = tid; b < nFull; b += NT) scoreBlock(b, false);
+            if (nFull < nBlk && tid == (nFull % NT)) scoreBlock(nFull, true);
+
+            if (laneReduce) {
+                if (P >= 8) {
+                    const int Q = P >> 3;   // lanes l and l' share groups iff l = l' (mod Q)
+                    for (int m = Q; m < 64; m <<= 1) {
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const int32_t s2 = __shfl_xor(accS[j], m, 64);
+                            const int32_t v2 = __shfl_xor(accV[j], m, 64);
+                            keepBetter(accS[j], accV[j], s2, v2);
+                        }
+                    }
+                    if (lane < Q) {
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            part[wave * pstride + 8 * lane + j] = packKey(accS[j], accV[j]);
+                    }
+                } else {
+                    // P in {1, 2, 4}: slot j feeds group j mod P.
+                    if (P <= 4) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) keepBetter(accS[j], accV[j], accS[j + 4], accV[j + 4]);
+                    }
+                    if (P <= 2) {
+#pragma unroll
+                        for (int j = 0; j < 2; ++j) keepBetter(accS[j], accV[j], accS[j + 2], accV[j + 2]);
+                    }
+                    if (P == 1) keepBetter(accS[0], accV[0], accS[1], accV[1]);
+#pragma unroll
+                    for (int m = 1; m < 64; m <<= 1) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int32_t s2 = __shfl_xor(accS[j], m, 64);
+                            const int32_t v2 = __shfl_xor(accV[j], m, 64);
+                            keepBetter(accS[j], accV[j], s2, v2);
+                        }
+                    }
+                    if (lane == 0) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (j < P) part[wave * pstride + j] = packKey(accS[j], accV[j]);
+                    }
+                }
+            }
+            __syncthreads();   // B1: all partials / slots written
+
+            // ---------------- RESOLVE: winner of each group ----------------
+            for (int g = tid; g < P; g += NT) {
+                uint64_t key = 0ull;
+                // Every entry read here is cleared again, so the region is all
+                // zero between sweeps whichever reduction the next phase uses.
+                if (laneReduce) {
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) {
+                        const uint64_t k2 = part[w * pstride + g];
+                        part[w * pstride + g] = 0ull;
+                        key = k2 > key ? k2 : key;
+                    }
+                } else {
+                    key = part[g];
+                    part[g] = 0ull;
+                }
+                int32_t pick = -1, sp = 0, dl = 0;
+                if (key != 0ull) {
+                    pick = static_cast<int32_t>(~static_cast<uint32_t>(key));
+                    const uint32_t w = ldsWord<kBits>(wl, pick);
+                    sp = (w & F::kSpin) ? 1 : -1;
+                    dl = wordDelta<kBits>(w);
+                }
+                pk[g] = pick;
+                ps[g] = sp;
+                pd[g] = dl;
+            }
+            __syncthreads();   // B2: winners published
+
+            // ---------------- FLIP ----------------
+            int64_t dsum = 0;
+            {
+                int lgL = 0;
+                if (P < NT) lgL = 31 - __clz(NT / P);
+                const int L = 1 << lgL;
+                const int sub = tid & (L - 1);
+                for (int g = tid >> lgL; g < P; g += (NT >> lgL)) {
+                    const int32_t b = pk[g];
+                    if (b < 0) continue;
+                    const int32_t sb = ps[g];
+                    const uint32_t lo = offsets[b];
+                    const uint32_t hi = offsets[b + 1];
+                    for (uint32_t i = lo + sub; i < hi; i += L) {
+                        const int2 e = edges[i];
+                        const int x = e.x;
+                        const int32_t w2 = 2 * e.y;
+                        const int gx = pow2 ? (x & (P - 1)) : (x % P);
+                        // Earlier winner adjacent to b: the coupling term that
+                        // sequential group order would have seen in S[b].
+                        if (gx < g && pk[gx] == x)
+                            dsum -= static_cast<int64_t>(w2) * (sb * ps[gx]);
+                        ldsAddField<kBits>(wl, x, -w2 * sb);
+                    }
+                    if (sub == 0) {
+                        dsum += pd[g];
+                        ldsFlipSpin<kBits>(wl, b);
+                    }
+                }
+            }
+            dsum = waveSum64(dsum);
+            if (lane == 0) dpart[wave] = dsum;
+            __syncthreads();   // B3: spins, fields and partial sums complete
+
+            int64_t total = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) total += dpart[w];
+            cur += total;
+            if (cur > best) { best = cur; improved = 1; } else { improved = 0; }
+            ++step;
+        }
+
+        // The snapshot rides one sweep late in SELECT; an improving final
+        // sweep is captured here.
+        if (improved) {
+            for (int b = tid; b < nBlk; b += NT) {
+                Block8<kBits> blk;
+                blk.load(wl, b);
+                uint32_t m = blk.spinByte();
+                if (8 * b + 8 > n) m &= (1u << (n - 8 * b)) - 1u;
+                bl8[b] = static_cast<uint8_t>(m);
+            }
+        }
+        __syncthreads();
+        {
+            uint4* gw = reinterpret_cast<uint4*>(it.words);
+            const uint4* lw = reinterpret_cast<const uint4*>(wl);
+            for (int i = tid; i < wq; i += NT) gw[i] = lw[i];
+            uint4* gb = reinterpret_cast<uint4*>(it.bestbits);
+            const uint4* lb = reinterpret_cast<const uint4*>(bl);
+            for (int i = tid; i < bq; i += NT) gb[i] = lb[i];
+        }
+        if (tid == 0) {
+            st->cur = cur;
+            st->best = best;
+            st->step = step;
+            st->improved = improved;
+        }
+        __syncthreads();
+    }
+
+    // Last workgroup out resets the work counters, so the next launch (or
+    // graph replay) starts from zero without a memset.
+    if (tid == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(&sched[1], 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(&sched[0], 0u);
+            atomicExch(&sched[1], 0u);
+        }
+    }
+}
+
+// Packs initial spins into state words: S[v] = sum_w w * spin[to] over v's
+// sorted row (src/anneal_kernels.cu:251-256), word = spin bit | S + bias;
+// best-spin bitmask = initial spins.  One thread per 32 vertices.
+template <int kBits>
+__global__ __launch_bounds__(256) void initWords(const InitItem* __restrict__ items, int nItems) {
+    using F = WordFmt<kBits>;
+    const InitItem& it = items[blockIdx.y];
+    const int n = it.n;
+    const int nGroups = (n + 31) >> 5;
+    const int2* edges = reinterpret_cast<const int2*>(it.edges);
+    for (int grp = blockIdx.x * blockDim.x + threadIdx.x; grp < nGroups;
+         grp += gridDim.x * blockDim.x) {
+        uint32_t bits = 0;
+        uint32_t packed[kBits];
+#pragma unroll
+        for (int q = 0; q < kBits; ++q) packed[q] = 0u;
+#pragma unroll 4
+        for (int k = 0; k < 32; ++k) {
+            const int v = 32 * grp + k;
+            uint32_t word = 0u;
+            if (v < n) {
+                int32_t S = 0;
+                for (uint32_t i = it.offsets[v]; i < it.offsets[v + 1]; ++i) {
+                    const int2 e = edges[i];
+                    S += e.y * static_cast<int32_t>(it.spins[e.x]);
+                }
+                const bool up = it.spins[v] > 0;
+                word = (up ? F::kSpin : 0u) | static_cast<uint32_t>(S + F::kBias);
+                bits |= (up ? 1u : 0u) << k;
+            }
+            if constexpr (kBits == 32) {
+                packed[k] = word;
+            } else {
+                constexpr int per = 32 / kBits;
+                packed[k / per] |= word << (kBits * (k % per));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kBits; ++q) it.words[grp * kBits + q] = packed[q];
+        it.bestbits[grp] = bits;
+    }
+}
+
+}  // namespace dev
+
+namespace launch {
+
+using dev::ChainItem;
+using dev::InitItem;
+using dev::LdsPlan;
+using dev::kThreads;
+
+template <int kBits, bool kMul24>
+static hipError_t launchAnneal(const ChainItem* items, int nItems, uint32_t* sched,
+                               const LdsPlan& plan, int grid, uint64_t sliceSweeps,
+                               hipStream_t s) {
+    auto kern = dev::annealSlice<kBits, kMul24, kThreads>;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(plan.total));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), plan.total, s, items, nItems, sched, plan,
+                       sliceSweeps);
+    return hipGetLastError();
+}
+
+hipError_t anneal(int bits, bool mul24, const ChainItem* items, int nItems, uint32_t* sched,
+                  const LdsPlan& plan, int grid, uint64_t sliceSweeps, hipStream_t s) {
+    switch (bits) {
+        case 8:  return mul24 ? launchAnneal<8, true>(items, nItems, sched, plan, grid, sliceSweeps, s)
+                              : launchAnneal<8, false>(items, nItems, sched, plan, grid, sliceSweeps, s);
+        case 16: return mul24 ? launchAnneal<16, true>(items, nItems, sched, plan, grid, sliceSweeps, s)
+                              : launchAnneal<16, false>(items, nItems, sched, plan, grid, sliceSweeps, s);
+        case 32: return mul24 ? launchAnneal<32, true>(items, nItems, sched, plan, grid, sliceSweeps, s)
+                              : launchAnneal<32, false>(items, nItems, sched, plan, grid, sliceSweeps, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int kBits, bool kMul24>
+static int occupancyOf(size_t lds) {
+    int blocks = 0;
+    auto kern = dev::annealSlice<kBits, kMul24, kThreads>;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds)) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, kThreads, lds) != hipSuccess)
+        return 0;
+    return blocks;
+}
+
+int maxActiveBlocks(int bits, bool mul24, size_t lds) {
+    switch (bits) {
+        case 8:  return mul24 ? occupancyOf<8, true>(lds) : occupancyOf<8, false>(lds);
+        case 16: return mul24 ? occupancyOf<16, true>(lds) : occupancyOf<16, false>(lds);
+        case 32: return mul24 ? occupancyOf<32, true>(lds) : occupancyOf<32, false>(lds);
+        default: return 0;
+    }
+}
+
+hipError_t initWords(int bits, const InitItem* items, int nItems, int maxN, hipStream_t s) {
+    if (nItems <= 0) return hipSuccess;
+    const int groups = (maxN + 31) / 32;
+    const int bx = (groups + 255) / 256;
+    dim3 grid(bx > 0 ? bx : 1, nItems);
+    switch (bits) {
+        case 8:  hipLaunchKernelGGL(dev::initWords<8>, grid, dim3(256), 0, s, items, nItems); break;
+        case 16: hipLaunchKernelGGL(dev::initWords<16>, grid, dim3(256), 0, s, items, nItems); break;
+        case 32: hipLaunchKernelGGL(dev::initWords<32>, grid, dim3(256), 0, s, items, nItems); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace launch
+}  // namespace rfmf

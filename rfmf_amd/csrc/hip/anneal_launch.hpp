@@ -1,0 +1,86 @@
+// anneal_launch.hpp -- host-callable launchers for the annealing kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "anneal_device.hpp"
+
+namespace rfmf {
+namespace dev {
+
+// Threads per workgroup of the slice kernel (4 wave64).
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+// Byte offsets of the LDS regions of one workgroup (host-computed from the
+// largest graph and the largest P of the launch).
+struct LdsPlan {
+    uint32_t wordsOff;    // packed words of the resident chain
+    uint32_t bestOff;     // best-spin bitmask
+    uint32_t partOff;     // per-wave group partials / atomic slots (u64)
+    uint32_t pickOff;     // int32 winner id per group
+    uint32_t spinOff;     // int32 winner pre-flip spin per group
+    uint32_t deltaOff;    // int32 winner pre-flip delta per group
+    uint32_t lutOff;      // int32[16] Gumbel LUT
+    uint32_t dpartOff;    // int64 per-wave cut-delta partials
+    uint32_t miscOff;     // scheduler broadcast
+    uint32_t total;       // dynamic LDS bytes
+    int32_t partStride;   // u64 entries per wave in the partial table
+    int32_t slotWords;    // u64 entries of the partial/slot region
+};
+
+// One chain's initial-state build.
+struct InitItem {
+    const uint32_t* offsets;
+    const int32_t* edges;
+    const int8_t* spins;   // n initial spins, +/-1
+    uint32_t* words;
+    uint32_t* bestbits;
+    int32_t n;
+    int32_t pad;
+};
+
+inline uint32_t alignUp(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
+
+inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP) {
+    LdsPlan p{};
+    const int laneP = maxP < kMaxLaneReduceP ? maxP : kMaxLaneReduceP;
+    const int slots = (kWaves * laneP > maxP) ? kWaves * laneP : maxP;
+    uint32_t off = 0;
+    p.wordsOff = off; off += alignUp(static_cast<uint32_t>(wordBytes(maxN, bits)), 16);
+    p.bestOff = off;  off += alignUp(static_cast<uint32_t>(bestBytes(maxN)), 16);
+    p.partOff = off;  off += alignUp(static_cast<uint32_t>(slots) * 8u, 16);
+    p.pickOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
+    p.spinOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
+    p.deltaOff = off; off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
+    p.lutOff = off;   off += 64;
+    p.dpartOff = off; off += alignUp(kWaves * 8u, 16);
+    p.miscOff = off;  off += 16;
+    p.total = off;
+    p.partStride = laneP;
+    p.slotWords = slots;
+    return p;
+}
+
+}  // namespace dev
+
+namespace launch {
+
+// Persistent slice kernel over `nItems` chain items, each advanced by at most
+// `sliceSweeps` sweeps; `sched` is a 2-word device counter pair, zero before
+// the launch and left zero after it.
+hipError_t anneal(int bits, bool mul24, const dev::ChainItem* items, int nItems,
+                  uint32_t* sched, const dev::LdsPlan& plan, int grid, uint64_t sliceSweeps,
+                  hipStream_t s);
+
+// Resident workgroups per CU for a variant at a given dynamic LDS size.
+int maxActiveBlocks(int bits, bool mul24, size_t lds);
+
+// Initial words / best bits for `nItems` chains (maxN = largest n).
+hipError_t initWords(int bits, const dev::InitItem* items, int nItems, int maxN, hipStream_t s);
+
+}  // namespace launch
+}  // namespace rfmf

@@ -1,0 +1,422 @@
+// engine.cpp -- device runtime: graph upload, chain-state admission, the
+// slice launches and result collection.
+//
+// Host-side duties follow the reference driver (src/anneal_kernels.cu:611-831):
+// CSR flattening, the int32 gate verdict, initial cut per chain with the host
+// cut recompute, kernel-only event timing.  Unlike the reference, device
+// buffers are owned by a context and kept resident, chains of many graphs go
+// into one persistent launch, and a run can be cut into slices.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "../host/max_cut.hpp"
+
+namespace rfmf {
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        throw Error(Status::OutOfMemory, std::string(what) + ": " + hipGetErrorString(e));
+    throw Error(Status::Hip, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void DevBuf::alloc(size_t bytes) {
+    reset();
+    if (bytes == 0) bytes = 16;
+    hipCheck(hipMalloc(&p_, bytes), "hipMalloc");
+    bytes_ = bytes;
+}
+
+void DevBuf::reset() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    bytes_ = 0;
+}
+
+int wordBitsFor(int64_t B) {
+    if (B <= dev::WordFmt<8>::kMaxB) return 8;
+    if (B <= dev::WordFmt<16>::kMaxB) return 16;
+    if (B <= dev::WordFmt<32>::kMaxB) return 32;
+    return 0;
+}
+
+Context::Context(int device) : device_(device) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        throw Error(Status::NoDevice, "no HIP device visible");
+    if (device < 0 || device >= count)
+        throw Error(Status::NoDevice, "device ordinal " + std::to_string(device) + " out of range");
+    hipCheck(hipSetDevice(device_), "hipSetDevice");
+    hipCheck(hipGetDeviceProperties(&props_, device_), "hipGetDeviceProperties");
+    hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hipCheck(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreate");
+    sched_.alloc(64 * sizeof(uint32_t));
+    hipCheck(hipMemset(sched_.get(), 0, sched_.bytes()), "hipMemset");
+}
+
+Context::~Context() {
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (side_) (void)hipStreamSynchronize(side_);
+    graphs_.clear();
+    sched_.reset();
+    if (stream_) (void)hipStreamDestroy(stream_);
+    if (side_) (void)hipStreamDestroy(side_);
+}
+
+int32_t Context::uploadGraph(const uint32_t* offsets, const int32_t* to, const int32_t* w,
+                             int64_t n, int64_t m2) {
+    if (!offsets || n < 0 || m2 < 0 || (m2 > 0 && (!to || !w)))
+        throw Error(Status::InvalidArg, "uploadGraph: null CSR array or negative size");
+    if (n >= (int64_t{1} << 31) || m2 >= (int64_t{1} << 32))
+        throw Error(Status::Unsupported, "uploadGraph: graph exceeds 32-bit ids/offsets");
+    if (offsets[0] != 0 || static_cast<int64_t>(offsets[n]) != m2)
+        throw Error(Status::InvalidArg, "uploadGraph: offsets[0] must be 0 and offsets[n] == m2");
+    auto g = std::make_unique<DevGraph>();
+    g->n = n;
+    g->m2 = m2;
+    g->hOffsets.assign(offsets, offsets + n + 1);
+    g->hTo.assign(to, to + m2);
+    g->hW.assign(w, w + m2);
+    int64_t B = 0;
+    for (int64_t v = 0; v < n; ++v) {
+        if (offsets[v + 1] < offsets[v]) throw Error(Status::InvalidArg, "uploadGraph: offsets not monotone");
+        int64_t s = 0;
+        for (uint32_t i = offsets[v]; i < offsets[v + 1]; ++i) {
+            if (to[i] < 0 || to[i] >= n) throw Error(Status::InvalidArg, "uploadGraph: neighbour id out of range");
+            s += std::llabs(static_cast<long long>(w[i]));
+        }
+        B = std::max(B, s);
+    }
+    g->B = B;
+    std::vector<int32_t> inter(static_cast<size_t>(2 * m2));
+    for (int64_t i = 0; i < m2; ++i) {
+        inter[2 * i] = to[i];
+        inter[2 * i + 1] = w[i];
+    }
+    hipCheck(hipSetDevice(device_), "hipSetDevice");
+    g->offsets.alloc(static_cast<size_t>(n + 1) * sizeof(uint32_t));
+    g->edges.alloc(static_cast<size_t>(std::max<int64_t>(m2, 1)) * 8);
+    hipCheck(hipMemcpy(g->offsets.get(), offsets, (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice),
+             "upload offsets");
+    if (m2 > 0)
+        hipCheck(hipMemcpy(g->edges.get(), inter.data(), m2 * 8, hipMemcpyHostToDevice), "upload edges");
+    const int32_t id = nextGraph_++;
+    graphs_.emplace(id, std::move(g));
+    return id;
+}
+
+void Context::releaseGraph(int32_t id) {
+    const auto it = graphs_.find(id);
+    if (it == graphs_.end()) throw Error(Status::NotFound, "unknown graph id " + std::to_string(id));
+    hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    graphs_.erase(it);
+}
+
+const DevGraph* Context::graph(int32_t id) const {
+    const auto it = graphs_.find(id);
+    if (it == graphs_.end()) throw Error(Status::NotFound, "unknown graph id " + std::to_string(id));
+    return it->second.get();
+}
+
+int Context::occupancy(int bits, bool mul24, size_t lds) {
+    const uint64_t key = (static_cast<uint64_t>(bits) << 40) | (static_cast<uint64_t>(mul24) << 39) | lds;
+    const auto it = occCache_.find(key);
+    if (it != occCache_.end()) return it->second;
+    const int occ = launch::maxActiveBlocks(bits, mul24, lds);
+    occCache_[key] = occ;
+    return occ;
+}
+
+std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t stream) {
+    const DevGraph* g = spec.graph;
+    if (!g) throw Error(Status::InvalidArg, "job has no graph");
+    if (spec.chains <= 0) throw Error(Status::InvalidArg, "job needs at least one chain");
+    if (!spec.initialSpins || !spec.seeds || !spec.runEnd || !spec.runVal || !spec.chainRunOff ||
+        spec.nRuns < 1)
+        throw Error(Status::InvalidArg, "job is missing spins, seeds or beta tables");
+    if (spec.phases.size() != spec.iters.size())
+        throw Error(Status::InvalidArg, "phases and iters differ in length");
+    const int64_t n = g->n;
+    const int64_t C = spec.chains;
+
+    auto job = std::make_unique<ResidentJob>();
+    job->graph = g;
+    job->chains = C;
+
+    // Executed phases: P > 0 and at least one sweep (a P == 0 phase is
+    // skipped without advancing the sweep counter, src/anneal_kernels.cu:270-272).
+    std::vector<uint64_t> hPhaseEnd;
+    std::vector<uint32_t> hPhaseP;
+    uint64_t total = 0;
+    int maxP = 1;
+    for (size_t p = 0; p < spec.phases.size(); ++p) {
+        const uint64_t P = spec.phases[p];
+        if (P == 0 || spec.iters[p] == 0) continue;
+        if (P > static_cast<uint64_t>(dev::kMaxGroups))
+            throw Error(Status::Unsupported, "group count P=" + std::to_string(P) + " exceeds " +
+                                                 std::to_string(dev::kMaxGroups));
+        total += spec.iters[p];
+        if (total > static_cast<uint64_t>(kMaxTotalSteps))
+            throw Error(Status::StepCap, "sweeps per chain exceed the total-sweep cap");
+        hPhaseEnd.push_back(total);
+        hPhaseP.push_back(static_cast<uint32_t>(P));
+        maxP = std::max<int>(maxP, static_cast<int>(P));
+    }
+    if (hPhaseEnd.empty()) {   // nothing executes; keep one dummy phase for the item layout
+        hPhaseEnd.push_back(0);
+        hPhaseP.push_back(1);
+    }
+    job->totalSteps = total;
+    job->maxP = maxP;
+
+    // Per-chain table extents: a table runs from its offset to the next
+    // larger distinct offset (tables are concatenated).
+    std::vector<int64_t> starts(spec.chainRunOff, spec.chainRunOff + C);
+    for (int64_t o : starts)
+        if (o < 0 || o >= spec.nRuns) throw Error(Status::InvalidArg, "chain_run_off out of range");
+    std::vector<int64_t> uniq = starts;
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    auto tableLen = [&](int64_t off) {
+        const auto it = std::upper_bound(uniq.begin(), uniq.end(), off);
+        return (it == uniq.end() ? spec.nRuns : *it) - off;
+    };
+    int64_t betaMax = 0;
+    for (int64_t off : uniq) {
+        const int64_t len = tableLen(off);
+        for (int64_t r = off; r < off + len; ++r) {
+            betaMax = std::max<int64_t>(betaMax, std::llabs(static_cast<long long>(spec.runVal[r])));
+            if (r > off && spec.runEnd[r] < spec.runEnd[r - 1])
+                throw Error(Status::InvalidArg, "beta run ends must be non-decreasing");
+        }
+        if (static_cast<uint64_t>(spec.runEnd[off + len - 1]) < total)
+            throw Error(Status::InvalidArg, "a beta table ends before the last sweep");
+    }
+    if (spec.gateBetaMax >= 0) betaMax = spec.gateBetaMax;
+
+    const int64_t gateB = std::max<int64_t>(spec.gateB, g->B);
+    if (!scoreSafeInt32ForBeta(betaMax, gateB))
+        throw Error(Status::Unsupported,
+                    "the int32 score gate fails (beta_max*B + |gumbel|max >= INT32_MAX); the int64 "
+                    "two-pass score path is not implemented");
+    job->bits = wordBitsFor(g->B);
+    if (job->bits == 0)
+        throw Error(Status::Unsupported, "max weighted degree >= 2^30 is not supported");
+    job->mul24 = betaMax < (int64_t{1} << 23) && g->B < (int64_t{1} << 23);
+    const dev::LdsPlan plan = dev::makeLdsPlan(n, job->bits, maxP);
+    if (static_cast<size_t>(plan.total) > props_.sharedMemPerBlock)
+        throw Error(Status::Unsupported, "chain state of n=" + std::to_string(n) + " (" +
+                                             std::to_string(plan.total) +
+                                             " B) exceeds the LDS per workgroup");
+
+    for (int64_t i = 0; i < C * n; ++i)
+        if (spec.initialSpins[i] != 1 && spec.initialSpins[i] != -1)
+            throw Error(Status::InvalidArg, "initial spins must be +1 or -1");
+
+    hipCheck(hipSetDevice(device_), "hipSetDevice");
+    job->wordStride = dev::wordBytes(n, job->bits);
+    job->bestStride = dev::bestBytes(n);
+    job->words.alloc(static_cast<size_t>(C * job->wordStride));
+    job->best.alloc(static_cast<size_t>(C * job->bestStride));
+    job->states.alloc(static_cast<size_t>(C) * sizeof(dev::ChainState));
+    job->runEnd.alloc(static_cast<size_t>(spec.nRuns) * sizeof(int64_t));
+    job->runVal.alloc(static_cast<size_t>(spec.nRuns) * sizeof(int32_t));
+    job->phaseEnd.alloc(hPhaseEnd.size() * sizeof(uint64_t));
+    job->phaseP.alloc(hPhaseP.size() * sizeof(uint32_t));
+    job->footprint = static_cast<int64_t>(job->words.bytes() + job->best.bytes() + job->states.bytes() +
+                                          job->runEnd.bytes() + job->runVal.bytes() +
+                                          job->phaseEnd.bytes() + job->phaseP.bytes());
+
+    hipCheck(hipMemcpyAsync(job->runEnd.get(), spec.runEnd, spec.nRuns * sizeof(int64_t),
+                            hipMemcpyHostToDevice, stream), "upload beta ends");
+    hipCheck(hipMemcpyAsync(job->runVal.get(), spec.runVal, spec.nRuns * sizeof(int32_t),
+                            hipMemcpyHostToDevice, stream), "upload beta values");
+    hipCheck(hipMemcpyAsync(job->phaseEnd.get(), hPhaseEnd.data(), hPhaseEnd.size() * sizeof(uint64_t),
+                            hipMemcpyHostToDevice, stream), "upload phases");
+    hipCheck(hipMemcpyAsync(job->phaseP.get(), hPhaseP.data(), hPhaseP.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, stream), "upload phases");
+
+    // Initial state: words and best bits on the device, the initial cut of
+    // every chain by the host recompute (src/anneal_kernels.cu:754-764).
+    DevBuf spins(static_cast<size_t>(C * n));
+    hipCheck(hipMemcpyAsync(spins.get(), spec.initialSpins, C * n, hipMemcpyHostToDevice, stream),
+             "upload spins");
+    std::vector<dev::ChainState> hState(static_cast<size_t>(C));
+    for (int64_t c = 0; c < C; ++c) {
+        const int64_t cut = anneallib::solver::cutValueRaw(g->hOffsets.data(), g->hTo.data(), g->hW.data(),
+                                                           n, spec.initialSpins + c * n);
+        hState[c] = dev::ChainState{cut, cut, 0, 0, 0};
+    }
+    hipCheck(hipMemcpyAsync(job->states.get(), hState.data(), C * sizeof(dev::ChainState),
+                            hipMemcpyHostToDevice, stream), "upload chain state");
+    std::vector<dev::InitItem> init(static_cast<size_t>(C));
+    for (int64_t c = 0; c < C; ++c) {
+        init[c].offsets = g->offsets.as<uint32_t>();
+        init[c].edges = g->edges.as<int32_t>();
+        init[c].spins = spins.as<int8_t>() + c * n;
+        init[c].words = reinterpret_cast<uint32_t*>(job->words.as<uint8_t>() + c * job->wordStride);
+        init[c].bestbits = reinterpret_cast<uint32_t*>(job->best.as<uint8_t>() + c * job->bestStride);
+        init[c].n = static_cast<int32_t>(n);
+        init[c].pad = 0;
+    }
+    DevBuf dInit(init.size() * sizeof(dev::InitItem));
+    hipCheck(hipMemcpyAsync(dInit.get(), init.data(), dInit.bytes(), hipMemcpyHostToDevice, stream),
+             "upload init items");
+    // Chain c of a grid-y launch: gridDim.y is limited to 65535 per launch.
+    for (int64_t c0 = 0; c0 < C; c0 += 65535) {
+        const int cnt = static_cast<int>(std::min<int64_t>(65535, C - c0));
+        hipCheck(launch::initWords(job->bits, dInit.as<dev::InitItem>() + c0, cnt, static_cast<int>(n), stream),
+                 "initWords");
+    }
+    hipCheck(hipStreamSynchronize(stream), "admission");
+
+    job->items.resize(static_cast<size_t>(C));
+    job->stepsDone.assign(static_cast<size_t>(C), 0);
+    for (int64_t c = 0; c < C; ++c) {
+        dev::ChainItem& it = job->items[c];
+        std::memset(&it, 0, sizeof(it));
+        it.offsets = g->offsets.as<uint32_t>();
+        it.edges = g->edges.as<int32_t>();
+        it.words = reinterpret_cast<uint32_t*>(job->words.as<uint8_t>() + c * job->wordStride);
+        it.bestbits = reinterpret_cast<uint32_t*>(job->best.as<uint8_t>() + c * job->bestStride);
+        it.st = job->states.as<dev::ChainState>() + c;
+        it.betaEnd = job->runEnd.as<int64_t>() + starts[c];
+        it.betaVal = job->runVal.as<int32_t>() + starts[c];
+        it.nRuns = static_cast<int32_t>(tableLen(starts[c]));
+        it.phaseEnd = job->phaseEnd.as<uint64_t>();
+        it.phaseP = job->phaseP.as<uint32_t>();
+        it.nPhases = static_cast<int32_t>(hPhaseEnd.size());
+        it.key = spec.seeds[c];
+        it.stepEnd = total;
+        it.n = static_cast<int32_t>(n);
+    }
+    return job;
+}
+
+float Context::runSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs) {
+    SliceHandle h = launchSlice(jobs, knobs);
+    return waitSlice(h);
+}
+
+Context::SliceHandle Context::launchSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs) {
+    hipCheck(hipSetDevice(device_), "hipSetDevice");
+    SliceHandle h;
+    h.jobs = jobs;
+    // Group unfinished chains by kernel variant; items sorted by descending n
+    // so the persistent workgroups take the longest chains first.
+    struct Group {
+        std::vector<dev::ChainItem> items;
+        int64_t maxN = 0;
+        int maxP = 1;
+    };
+    std::vector<Group> groups(6);
+    auto gidx = [](int bits, bool mul24) { return (bits == 8 ? 0 : bits == 16 ? 1 : 2) * 2 + (mul24 ? 1 : 0); };
+    for (ResidentJob* j : jobs) {
+        Group& g = groups[gidx(j->bits, j->mul24)];
+        for (size_t c = 0; c < j->items.size(); ++c) {
+            if (j->stepsDone[c] >= j->totalSteps) continue;
+            g.items.push_back(j->items[c]);
+            g.maxN = std::max<int64_t>(g.maxN, j->graph->n);
+            g.maxP = std::max(g.maxP, j->maxP);
+        }
+    }
+    const uint64_t slice = knobs.sliceSweeps > 0 ? static_cast<uint64_t>(knobs.sliceSweeps) : ~uint64_t{0};
+    h.slice = slice;
+    std::vector<DevBuf>& itemBufs = h.itemBufs;
+    hipCheck(hipEventCreate(&h.ev0), "hipEventCreate");
+    hipCheck(hipEventCreate(&h.ev1), "hipEventCreate");
+    hipCheck(hipEventRecord(h.ev0, stream_), "hipEventRecord");
+    int slot = 0;
+    for (int gi = 0; gi < 6; ++gi) {
+        Group& g = groups[gi];
+        if (g.items.empty()) continue;
+        std::stable_sort(g.items.begin(), g.items.end(),
+                         [](const dev::ChainItem& a, const dev::ChainItem& b) { return a.n > b.n; });
+        const int bits = (gi / 2 == 0) ? 8 : (gi / 2 == 1) ? 16 : 32;
+        const bool mul24 = (gi % 2) == 1;
+        const dev::LdsPlan plan = dev::makeLdsPlan(g.maxN, bits, g.maxP);
+        if (static_cast<size_t>(plan.total) > props_.sharedMemPerBlock)
+            throw Error(Status::Unsupported, "combined launch needs " + std::to_string(plan.total) +
+                                                 " B of LDS per workgroup");
+        const int occ = std::max(1, occupancy(bits, mul24, plan.total));
+        int grid = occ * props_.multiProcessorCount;
+        if (knobs.gridCap > 0) grid = std::min(grid, knobs.gridCap);
+        grid = static_cast<int>(std::min<int64_t>(grid, static_cast<int64_t>(g.items.size())));
+        itemBufs.emplace_back(g.items.size() * sizeof(dev::ChainItem));
+        hipCheck(hipMemcpyAsync(itemBufs.back().get(), g.items.data(), itemBufs.back().bytes(),
+                                hipMemcpyHostToDevice, stream_), "upload items");
+        hipCheck(launch::anneal(bits, mul24, itemBufs.back().as<dev::ChainItem>(),
+                                static_cast<int>(g.items.size()), sched_.as<uint32_t>() + 2 * slot, plan,
+                                grid, slice, stream_), "annealSlice launch");
+        ++slot;
+        h.empty = false;
+    }
+    hipCheck(hipEventRecord(h.ev1, stream_), "hipEventRecord");
+    return h;
+}
+
+float Context::waitSlice(SliceHandle& h) {
+    float ms = 0.0f;
+    if (h.ev1) {
+        hipCheck(hipEventSynchronize(h.ev1), "annealSlice");
+        hipCheck(hipEventElapsedTime(&ms, h.ev0, h.ev1), "hipEventElapsedTime");
+        (void)hipEventDestroy(h.ev0);
+        (void)hipEventDestroy(h.ev1);
+        h.ev0 = h.ev1 = nullptr;
+    }
+    h.itemBufs.clear();
+    for (ResidentJob* j : h.jobs)
+        for (auto& s : j->stepsDone) s = (j->totalSteps - s > h.slice) ? s + h.slice : j->totalSteps;
+    h.jobs.clear();
+    return ms;
+}
+
+float Context::runToCompletion(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs) {
+    float total = 0.0f;
+    for (;;) {
+        bool pending = false;
+        for (const ResidentJob* j : jobs)
+            for (uint64_t s : j->stepsDone) pending |= s < j->totalSteps;
+        if (!pending) break;
+        total += runSlice(jobs, knobs);
+    }
+    return total;
+}
+
+void Context::collect(const ResidentJob& job, const JobSpec& spec, hipStream_t stream) {
+    const int64_t C = job.chains;
+    const int64_t n = job.graph->n;
+    std::vector<uint8_t> hw(static_cast<size_t>(C * job.wordStride));
+    std::vector<uint8_t> hb(static_cast<size_t>(C * job.bestStride));
+    std::vector<dev::ChainState> hs(static_cast<size_t>(C));
+    hipCheck(hipMemcpyAsync(hw.data(), job.words.get(), hw.size(), hipMemcpyDeviceToHost, stream), "download words");
+    hipCheck(hipMemcpyAsync(hb.data(), job.best.get(), hb.size(), hipMemcpyDeviceToHost, stream), "download best");
+    hipCheck(hipMemcpyAsync(hs.data(), job.states.get(), hs.size() * sizeof(dev::ChainState),
+                            hipMemcpyDeviceToHost, stream), "download state");
+    hipCheck(hipStreamSynchronize(stream), "download");
+    const int bytesPerWord = job.bits / 8;
+    for (int64_t c = 0; c < C; ++c) {
+        const uint8_t* w = hw.data() + c * job.wordStride;
+        const uint8_t* b = hb.data() + c * job.bestStride;
+        if (spec.endSpins) {
+            int8_t* out = spec.endSpins + c * n;
+            for (int64_t v = 0; v < n; ++v) {
+                // little-endian: the spin bit is the top bit of the word's last byte
+                const uint8_t top = w[v * bytesPerWord + bytesPerWord - 1];
+                out[v] = (top & 0x80u) ? int8_t{1} : int8_t{-1};
+            }
+        }
+        if (spec.bestSpins) {
+            int8_t* out = spec.bestSpins + c * n;
+            for (int64_t v = 0; v < n; ++v) out[v] = ((b[v >> 3] >> (v & 7)) & 1u) ? int8_t{1} : int8_t{-1};
+        }
+        if (spec.bestVals) spec.bestVals[c] = hs[c].best;
+        if (spec.finalVals) spec.finalVals[c] = hs[c].cur;
+    }
+}
+
+}  // namespace rfmf

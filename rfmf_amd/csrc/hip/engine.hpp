@@ -1,0 +1,154 @@
+// engine.hpp -- device runtime of the annealer: resident graphs, chain
+// state in HBM, and the persistent slice launches that advance any set of
+// chains (one graph or many) by a number of sweeps.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../host/host_common.hpp"
+#include "anneal_device.hpp"
+#include "anneal_launch.hpp"
+
+namespace rfmf {
+
+void hipCheck(hipError_t e, const char* what);
+
+// RAII device buffer.
+class DevBuf {
+public:
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) { alloc(bytes); }
+    ~DevBuf() { reset(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; o.bytes_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { reset(); p_ = o.p_; bytes_ = o.bytes_; o.p_ = nullptr; o.bytes_ = 0; }
+        return *this;
+    }
+    void alloc(size_t bytes);
+    void reset();
+    template <class T> T* as() const { return static_cast<T*>(p_); }
+    void* get() const { return p_; }
+    size_t bytes() const { return bytes_; }
+
+private:
+    void* p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+struct DevGraph {
+    DevBuf offsets;   // u32[n+1]
+    DevBuf edges;     // int2[m2] {to, weight}
+    int64_t n = 0;
+    int64_t m2 = 0;
+    int64_t B = 0;    // max weighted degree (computed at upload)
+    std::vector<uint32_t> hOffsets;   // host copy for the initial cut
+    std::vector<int32_t> hTo, hW;
+};
+
+// Host description of one job (a set of chains of one graph).
+struct JobSpec {
+    const DevGraph* graph = nullptr;
+    int64_t chains = 0;
+    const int8_t* initialSpins = nullptr;   // chains x n
+    const uint64_t* seeds = nullptr;         // chains
+    const int64_t* runEnd = nullptr;
+    const int32_t* runVal = nullptr;
+    const int64_t* chainRunOff = nullptr;    // chains
+    int64_t nRuns = 0;
+    std::vector<uint64_t> phases, iters;     // as given (P == 0 phases included)
+    int64_t gateB = -1;                      // B for the int32 gate (-1: graph's B)
+    int32_t gateBetaMax = -1;                // |beta| max for the gate (-1: from tables)
+    int8_t* endSpins = nullptr;
+    int8_t* bestSpins = nullptr;
+    int64_t* bestVals = nullptr;
+    int64_t* finalVals = nullptr;
+};
+
+// Chain state of one admitted job, resident in HBM.
+struct ResidentJob {
+    const DevGraph* graph = nullptr;
+    int64_t chains = 0;
+    int bits = 32;
+    bool mul24 = false;
+    int maxP = 1;
+    uint64_t totalSteps = 0;   // executed sweeps per chain
+    DevBuf words, best, states, runEnd, runVal, phaseEnd, phaseP;
+    int64_t wordStride = 0, bestStride = 0;
+    std::vector<dev::ChainItem> items;   // one per chain (device pointers filled)
+    std::vector<uint64_t> stepsDone;     // host mirror of each chain's step after the last sync
+    int64_t footprint = 0;
+};
+
+struct LaunchKnobs {
+    int64_t sliceSweeps = 0;   // 0: run to completion in one launch
+    int gridCap = 0;
+    bool hipGraph = false;
+};
+
+class Context {
+public:
+    explicit Context(int device);
+    ~Context();
+
+    int device() const { return device_; }
+    hipStream_t stream() const { return stream_; }
+    hipStream_t sideStream() const { return side_; }
+    const hipDeviceProp_t& props() const { return props_; }
+
+    int32_t uploadGraph(const uint32_t* offsets, const int32_t* to, const int32_t* w, int64_t n,
+                        int64_t m2);
+    void releaseGraph(int32_t id);
+    const DevGraph* graph(int32_t id) const;
+
+    // Allocates and initialises a job's chain state on `stream` (initial
+    // words from the spins, state {cut, cut, 0, 0}).  Validates the job.
+    std::unique_ptr<ResidentJob> admit(const JobSpec& spec, hipStream_t stream);
+    // Advances every unfinished chain of `jobs` by up to `sliceSweeps` sweeps
+    // (all of them when 0), as few launches as there are kernel variants.
+    // Returns the kernel-only milliseconds measured with events on stream().
+    float runSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs);
+    // The same, split: enqueue the launches, then wait for them (the host can
+    // admit new work on the side stream in between).
+    struct SliceHandle {
+        std::vector<DevBuf> itemBufs;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        std::vector<ResidentJob*> jobs;
+        uint64_t slice = 0;
+        bool empty = true;
+    };
+    SliceHandle launchSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs);
+    float waitSlice(SliceHandle& h);
+    // Runs jobs to completion (slices repeated) and returns kernel ms.
+    float runToCompletion(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs);
+    // Copies a finished job's results to the spec's outputs.
+    void collect(const ResidentJob& job, const JobSpec& spec, hipStream_t stream);
+
+    // Autotuned knobs cache (by word bits and LDS bytes bucket).
+    int occupancy(int bits, bool mul24, size_t lds);
+
+    std::mutex& mutex() { return mu_; }
+
+private:
+    int device_;
+    hipStream_t stream_ = nullptr;
+    hipStream_t side_ = nullptr;
+    hipDeviceProp_t props_{};
+    DevBuf sched_;   // 2 x u32 work counters, zero between launches
+    std::unordered_map<int32_t, std::unique_ptr<DevGraph>> graphs_;
+    int32_t nextGraph_ = 1;
+    std::unordered_map<uint64_t, int> occCache_;
+    std::mutex mu_;
+};
+
+int wordBitsFor(int64_t B);   // 8 / 16 / 32, or 0 if unsupported
+
+}  // namespace rfmf

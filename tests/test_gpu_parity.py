@@ -1,0 +1,113 @@
+"""HIP path vs the CPU oracle, bit-exact, through the C ABI.
+
+Each case runs C chains on the GPU and the same chains in the oracle (the
+sequential group-order restatement of the reference kernel) and requires
+identical final spins, best spins, incremental final and best cuts, and
+incremental cuts equal to the host recompute of the returned spins."""
+import numpy as np
+import pytest
+
+import rfmf_amd as R
+from rfmf_amd import workloads as W
+from conftest import run_oracle_chains
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name, csr, seeds, beta, phases, iters, B=None):
+    print(f"[gpu-parity] {name}: n={csr.n} m2={csr.m2} chains={len(seeds)} phases={list(phases)} "
+          f"iters={list(iters)}", flush=True)
+    n = csr.n
+    init = np.stack([R.generate_random_spins(s, n) for s in seeds])
+    solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+    B = R.max_weighted_degree(csr) if B is None else B
+    res = R.anneal_gpu_tables(csr, B, init, beta, phases, iters, solver)
+    end, best, cv, bv = run_oracle_chains(csr, init, solver, beta, phases, iters)
+    for c in range(len(seeds)):
+        assert np.array_equal(res.end_spins[c], end[c]), f"{name} chain {c}: final spins differ"
+        assert np.array_equal(res.best_spins[c], best[c]), f"{name} chain {c}: best spins differ"
+        assert res.kernel_final_vals[c] == cv[c], f"{name} chain {c}: final cut differs"
+        assert res.kernel_best_vals[c] == bv[c], f"{name} chain {c}: best cut differs"
+        assert R.cut_value_csr(csr, res.end_spins[c]) == res.kernel_final_vals[c]
+        assert R.cut_value_csr(csr, res.best_spins[c]) == res.kernel_best_vals[c]
+    return res
+
+
+def _log_tables(t0s, iters):
+    return R.build_beta_run_tables(t0s, int(sum(iters)))
+
+
+def test_g11_four_chains_log_schedule(gset):
+    csr = gset("G11")
+    phases, iters = [8, 4, 2, 1], [150] * 4
+    _case("G11", csr, [42, 7, 42, 9], _log_tables([105, 105, 90, 120], iters), phases, iters)
+
+
+def test_g1_dense_u16_words(gset):
+    csr = gset("G1")
+    phases, iters = [64, 32, 16, 8, 4, 2, 1], [120] * 7
+    _case("G1", csr, [42, 43, 42], _log_tables([105, 105, 90], iters), phases, iters)
+
+
+def test_g62_full_p_ladder(gset):
+    csr = gset("G62")
+    phases, iters = [128, 64, 32, 16, 8, 4, 2, 1], [100] * 8
+    _case("G62", csr, [4247410429, 1771247888], _log_tables([35.0, 50.0], iters), phases, iters)
+
+
+def test_g70_sparse(gset):
+    csr = gset("G70")
+    phases, iters = [8, 2, 1], [150] * 3
+    _case("G70", csr, [13, 14], _log_tables([105, 105], iters), phases, iters)
+
+
+def test_g61_pm1(gset):
+    csr = gset("G61")
+    phases, iters = [64, 16], [150, 150]
+    _case("G61", csr, [9, 10], _log_tables([105, 95], iters), phases, iters)
+
+
+def test_er800_linear_schedule():
+    csr = W.erdos_renyi(800, 0.06, seed=1)
+    phases, iters = [128, 64, 32, 16, 8, 4, 2, 1], [250] * 8
+    beta = R.build_linear_beta_table(0.1, 3.0, sum(iters), 8)
+    _case("ER800-linear", csr, list(range(1, 9)), beta, phases, iters)
+
+
+@pytest.mark.parametrize("P", [3, 5, 12, 100, 1000, 1024, 2048])
+def test_general_group_counts(P):
+    # odd n (tail block), weights up to +/-5, P not a power of two / above
+    # the lane-reduce limit / above n: the LDS-atomic argmax path.
+    rng = np.random.default_rng(P)
+    n, m = 613, 2400
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    keep = u != v
+    w = rng.integers(1, 6, m) * np.where(rng.random(m) < 0.5, -1, 1)
+    csr = R.graph_from_edges(u[keep], v[keep], w[keep])
+    phases, iters = [P, 1], [90, 30]
+    _case(f"P={P}", csr, [5, 6, 7], _log_tables([40.0, 60.0, 80.0], iters), phases, iters)
+
+
+def test_wide_words_u32():
+    # a weighted degree above 16383 forces 32-bit state words
+    rng = np.random.default_rng(11)
+    n = 300
+    u = np.arange(n - 1)
+    v = u + 1
+    w = rng.integers(-3000, 3001, n - 1)
+    w[w == 0] = 1
+    u = np.concatenate([u, np.zeros(10, np.int64)])
+    v = np.concatenate([v, np.arange(2, 12)])
+    w = np.concatenate([w, np.full(10, 2000)])
+    csr = R.graph_from_edges(u, v, w)
+    assert R.max_weighted_degree(csr) > 16383
+    phases, iters = [16, 4, 1], [60, 60, 60]
+    beta = R.build_linear_beta_table(0.0, 0.02, sum(iters), 2)
+    _case("u32-words", csr, [3, 4], beta, phases, iters)
+
+
+def test_p_zero_phase_skipped(gset):
+    csr = gset("G11")
+    phases, iters = [8, 0, 4], [100, 50, 100]
+    _case("P0-phase", csr, [1, 2], _log_tables([100.0, 100.0], iters), phases, iters)

@@ -206,6 +206,14 @@ typedef struct rfmf_tuning {
 rfmf_status rfmf_batch_run(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs,
                            const rfmf_tuning* tuning, float* kernel_ms);
 
+/* rfmf_batch_run cut into timed windows: launch i advances every chain to
+ * sweep min(total, step_bounds[i]) (bounds ascending) and window_ms[i]
+ * receives its kernel-only time; after the last window the chains run to
+ * completion untimed.  Used to time a window of a run with the state already
+ * resident (e.g. after warmup sweeps). */
+rfmf_status rfmf_batch_run_windows(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs,
+                                   const int64_t* step_bounds, int64_t n_windows, float* window_ms);
+
 /* HBM bytes a job of this shape keeps resident (CSR + chain state + tables). */
 int64_t rfmf_footprint_bytes(int64_t n, int64_t m2, int64_t chains, int64_t beta_runs,
                              int64_t n_phases);

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "host_common.h"
+#include "philox.cuh"
 #include "rng.hpp"
 
 namespace {
@@ -80,6 +81,39 @@ int main(int argc, char** argv) {
     }
     if (cmd == "betafix" && argc == 4) {
         std::cout << betaFixedForStep(std::stoull(argv[2]), std::stod(argv[3])) << "\n";
+        return 0;
+    }
+    if (cmd == "philox" && argc == 8) {
+        gpu3::Philox4 c;
+        gpu3::Philox2 k;
+        for (int i = 0; i < 4; ++i) c.v[i] = static_cast<uint32_t>(std::stoul(argv[2 + i]));
+        for (int i = 0; i < 2; ++i) k.v[i] = static_cast<uint32_t>(std::stoul(argv[6 + i]));
+        const gpu3::Philox4 o = gpu3::philox4x32_10(c, k);
+        std::cout << o.v[0] << " " << o.v[1] << " " << o.v[2] << " " << o.v[3] << "\n";
+        return 0;
+    }
+    if (cmd == "gumbels" && argc == 5) {
+        // Gumbel of vertices [0, n) in 1-based sweep `step` of the chain keyed by `solverSeed`.
+        const uint64_t seed = std::stoull(argv[2]);
+        const uint64_t step = std::stoull(argv[3]);
+        const uint64_t n = std::stoull(argv[4]);
+        const gpu3::Philox2 key = gpu3::philoxKeyFromSeed(seed);
+        for (uint64_t v = 0; v < n; ++v) {
+            const gpu3::Philox4 blk = gpu3::philox4x32_10(gpu3::gumbelBlockCounter(step, v >> 3), key);
+            std::cout << (v ? " " : "")
+                      << gpu3::gumbelFromBitsG(gpu3::gumbelBitsFromBlock(blk, static_cast<uint32_t>(v & 7)));
+        }
+        std::cout << "\n";
+        return 0;
+    }
+    if (cmd == "gumbelbits") {
+        for (uint32_t r = 0; r < 2048; ++r) std::cout << (r ? " " : "") << gpu3::gumbelFromBitsG(r);
+        std::cout << "\n";
+        return 0;
+    }
+    if (cmd == "pack" && argc == 4) {
+        std::cout << gpu3::packScoreId(static_cast<int32_t>(std::stol(argv[2])), static_cast<int32_t>(std::stol(argv[3])))
+                  << "\n";
         return 0;
     }
     if (cmd == "cut" && argc == 4) {

@@ -349,6 +349,15 @@ class Context:
         check(lib().rfmf_batch_run(self._h, arr, len(jobs), C.byref(tun), C.byref(ms)))
         return ms.value
 
+    def batch_run_windows(self, jobs: list[tuple[int, Job]], bounds: list[int]) -> list[float]:
+        """Runs jobs with one timed launch per window [bounds[i-1], bounds[i])
+        of sweeps, then to completion; returns the windows' kernel ms."""
+        arr = (_capi.Job * len(jobs))(*[j.struct(gid) for gid, j in jobs])
+        b = np.ascontiguousarray(bounds, dtype=np.int64)
+        ms = (C.c_float * max(1, b.size))()
+        check(lib().rfmf_batch_run_windows(self._h, arr, len(jobs), _ptr(b, C.c_int64), b.size, ms))
+        return [ms[i] for i in range(b.size)]
+
 
 class Server:
     """Continuous-batching solve server over a Context."""

@@ -432,4 +432,36 @@ rfmf_status rfmf_batch_run(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs, 
     });
 }
 
+rfmf_status rfmf_batch_run_windows(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs,
+                                   const int64_t* step_bounds, int64_t n_windows, float* window_ms) {
+    return guarded([&] {
+        if (!ctx || n_jobs <= 0 || !jobs || n_windows < 0 || (n_windows > 0 && (!step_bounds || !window_ms)))
+            throw Error(Status::InvalidArg, "invalid batch or windows");
+        for (int64_t i = 0; i < n_windows; ++i)
+            if (step_bounds[i] < 0 || (i > 0 && step_bounds[i] < step_bounds[i - 1]))
+                throw Error(Status::InvalidArg, "window bounds must be ascending and non-negative");
+        std::lock_guard<std::mutex> lk(ctx->ctx->mutex());
+        rfmf::Context& c = *ctx->ctx;
+        std::vector<rfmf::JobSpec> specs;
+        for (int64_t i = 0; i < n_jobs; ++i) specs.push_back(specFromJob(jobs[i], c.graph(jobs[i].graph_id)));
+        std::vector<std::unique_ptr<rfmf::ResidentJob>> resident;
+        std::vector<rfmf::ResidentJob*> ptrs;
+        for (const auto& s : specs) {
+            resident.push_back(c.admit(s, c.stream()));
+            ptrs.push_back(resident.back().get());
+        }
+        // Every chain of a fresh batch starts at sweep 0, so window i is one
+        // slice of (bound_i - bound_{i-1}) sweeps.
+        int64_t at = 0;
+        for (int64_t i = 0; i < n_windows; ++i) {
+            rfmf::LaunchKnobs k;
+            k.sliceSweeps = step_bounds[i] - at;
+            window_ms[i] = k.sliceSweeps > 0 ? c.runSlice(ptrs, k) : 0.0f;
+            at = step_bounds[i];
+        }
+        c.runToCompletion(ptrs, rfmf::LaunchKnobs{});
+        for (size_t i = 0; i < specs.size(); ++i) c.collect(*resident[i], specs[i], c.stream());
+    });
+}
+
 }  // extern "C"

@@ -64,7 +64,17 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-chains", type=int, default=32)
     ap.add_argument("--cpu-sweeps", type=int, default=4000)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel over BYTES (counter calibration) and exit")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import rfmf_amd as R
+        with R.Context(0) as ctx:
+            ms = ctx.copy_probe(args.copy_probe, iters=5)
+        log(f"[bench] copy probe {args.copy_probe} B: {ms:.3f} ms/launch, "
+            f"{2 * args.copy_probe / ms / 1e6:.1f} GB/s read+write")
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the annealer shards by chains and graphs with no cross-GPU "

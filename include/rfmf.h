@@ -174,6 +174,12 @@ rfmf_status rfmf_ctx_release_graph(rfmf_ctx* ctx, int32_t graph_id);
 rfmf_status rfmf_ctx_device_info(rfmf_ctx* ctx, int32_t* cu_count, int64_t* lds_per_block,
                                  int64_t* hbm_bytes, char* name, int32_t name_cap);
 
+/* Copies `bytes` (rounded down to 16) between two fresh device buffers with a
+ * plain streaming kernel `iters` times; returns the average kernel ms.  A
+ * known-size kernel for calibrating HBM traffic counters and measuring the
+ * achievable copy bandwidth (no reference counterpart). */
+rfmf_status rfmf_ctx_copy_probe(rfmf_ctx* ctx, int64_t bytes, int32_t iters, float* avg_ms);
+
 /* A job: `chains` chains of one resident graph with their own schedule. */
 typedef struct rfmf_job {
     int32_t graph_id;

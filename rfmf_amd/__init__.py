@@ -330,6 +330,12 @@ class Context:
         check(lib().rfmf_ctx_device_info(self._h, C.byref(cu), C.byref(lds), C.byref(hbm), name, 64))
         return {"cu": cu.value, "lds_per_block": lds.value, "hbm_bytes": hbm.value, "arch": name.value.decode()}
 
+    def copy_probe(self, nbytes: int, iters: int = 10) -> float:
+        """Average ms of a plain device copy of `nbytes` (read + write)."""
+        ms = C.c_float()
+        check(lib().rfmf_ctx_copy_probe(self._h, int(nbytes), int(iters), C.byref(ms)))
+        return ms.value
+
     def upload_graph(self, csr: AnnealCsr) -> int:
         gid = C.c_int32()
         check(lib().rfmf_ctx_upload_graph(self._h, C.byref(csr.view()), C.byref(gid)))

@@ -82,6 +82,7 @@ SIGNATURES = {
     "rfmf_ctx_upload_graph": (C.c_int, [C.c_void_p, C.POINTER(Csr), i32p]),
     "rfmf_ctx_release_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "rfmf_ctx_device_info": (C.c_int, [C.c_void_p, i32p, i64p, i64p, C.c_char_p, C.c_int32]),
+    "rfmf_ctx_copy_probe": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, f32p]),
     "rfmf_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int64, C.POINTER(Tuning), f32p]),
     "rfmf_batch_run_windows": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int64, i64p, C.c_int64, f32p]),
     "rfmf_footprint_bytes": (C.c_int64, [C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int64]),

@@ -483,9 +483,25 @@ __global__ __launch_bounds__(256) void initWords(const InitItem* __restrict__ it
     }
 }
 
+// Plain device copy (16 B per lane, grid-stride): the known-size reference
+// kernel that HBM traffic counters are calibrated against.
+__global__ __launch_bounds__(256) void streamCopy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                  size_t n16) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n16;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace dev
 
 namespace launch {
+
+hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    const size_t n16 = bytes / 16;
+    hipLaunchKernelGGL(dev::streamCopy, dim3(4096), dim3(256), 0, s, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), n16);
+    return hipGetLastError();
+}
 
 using dev::ChainItem;
 using dev::InitItem;

@@ -79,6 +79,9 @@ hipError_t anneal(int bits, bool mul24, const dev::ChainItem* items, int nItems,
 // Resident workgroups per CU for a variant at a given dynamic LDS size.
 int maxActiveBlocks(int bits, bool mul24, size_t lds);
 
+// Device-to-device copy of `bytes` (a multiple of 16) with a plain kernel.
+hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s);
+
 // Initial words / best bits for `nItems` chains (maxN = largest n).
 hipError_t initWords(int bits, const dev::InitItem* items, int nItems, int maxN, hipStream_t s);
 

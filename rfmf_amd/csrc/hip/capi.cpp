@@ -369,6 +369,31 @@ rfmf_status rfmf_ctx_device_info(rfmf_ctx* ctx, int32_t* cu_count, int64_t* lds_
     });
 }
 
+rfmf_status rfmf_ctx_copy_probe(rfmf_ctx* ctx, int64_t bytes, int32_t iters, float* avg_ms) {
+    return guarded([&] {
+        if (!ctx || bytes < 16 || iters <= 0 || !avg_ms) throw Error(Status::InvalidArg, "invalid probe");
+        std::lock_guard<std::mutex> lk(ctx->ctx->mutex());
+        rfmf::Context& c = *ctx->ctx;
+        const size_t b = static_cast<size_t>(bytes) & ~size_t{15};
+        rfmf::DevBuf src(b), dst(b);
+        rfmf::hipCheck(hipMemsetAsync(src.get(), 1, b, c.stream()), "hipMemsetAsync");
+        hipEvent_t e0, e1;
+        rfmf::hipCheck(hipEventCreate(&e0), "hipEventCreate");
+        rfmf::hipCheck(hipEventCreate(&e1), "hipEventCreate");
+        rfmf::hipCheck(rfmf::launch::streamCopy(src.get(), dst.get(), b, c.stream()), "streamCopy");
+        rfmf::hipCheck(hipEventRecord(e0, c.stream()), "hipEventRecord");
+        for (int i = 0; i < iters; ++i)
+            rfmf::hipCheck(rfmf::launch::streamCopy(src.get(), dst.get(), b, c.stream()), "streamCopy");
+        rfmf::hipCheck(hipEventRecord(e1, c.stream()), "hipEventRecord");
+        rfmf::hipCheck(hipEventSynchronize(e1), "streamCopy");
+        float ms = 0.0f;
+        rfmf::hipCheck(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_ms = ms / static_cast<float>(iters);
+    });
+}
+
 int64_t rfmf_footprint_bytes(int64_t n, int64_t m2, int64_t chains, int64_t beta_runs, int64_t n_phases) {
     const int bits = 32;   // conservative: the widest state word
     return (n + 1) * 4 + m2 * 8 + chains * (rfmf::dev::wordBytes(n, bits) + rfmf::dev::bestBytes(n) + 32) +

@@ -220,6 +220,20 @@ rfmf_status rfmf_batch_run(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs,
 rfmf_status rfmf_batch_run_windows(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs,
                                    const int64_t* step_bounds, int64_t n_windows, float* window_ms);
 
+/* Step-wise control of a batch: admit jobs once (chain state resident in
+ * HBM), advance every chain by `sweeps` per call (one persistent launch per
+ * kernel variant; with tuning->use_hip_graph the launch is captured once and
+ * replayed), and write the current state of every chain into the jobs'
+ * output buffers at any point (final spins = current spins, best spins and
+ * values = best so far).  The job output pointers must stay valid until
+ * rfmf_run_destroy. */
+typedef struct rfmf_run rfmf_run;
+rfmf_status rfmf_run_create(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs, rfmf_run** out);
+rfmf_status rfmf_run_advance(rfmf_run* run, int64_t sweeps, const rfmf_tuning* tuning, float* kernel_ms,
+                             int32_t* finished);
+rfmf_status rfmf_run_collect(rfmf_run* run);
+void rfmf_run_destroy(rfmf_run* run);
+
 /* HBM bytes a job of this shape keeps resident (CSR + chain state + tables). */
 int64_t rfmf_footprint_bytes(int64_t n, int64_t m2, int64_t chains, int64_t beta_runs,
                              int64_t n_phases);

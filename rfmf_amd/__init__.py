@@ -21,7 +21,7 @@ K_SOLVER_SEED_OFFSET = 0xC2B2AE3D27D4EB4F
 Q20 = 1 << 20
 
 __all__ = [
-    "AnnealCsr", "ReadStats", "BetaRunTables", "LaunchConfig", "AnnealResult", "Context", "Job",
+    "AnnealCsr", "ReadStats", "BetaRunTables", "LaunchConfig", "AnnealResult", "Context", "Job", "Run",
     "Server", "RfmfError", "read_instance_from_file", "graph_from_edges", "cut_value_csr",
     "max_weighted_degree", "anneal_score_safe_int32", "beta_fixed_for_step", "anneal_total_steps",
     "generate_random_spins", "build_beta_run_tables", "build_linear_beta_table", "anneal_gpu",
@@ -363,6 +363,38 @@ class Context:
         ms = (C.c_float * max(1, b.size))()
         check(lib().rfmf_batch_run_windows(self._h, arr, len(jobs), _ptr(b, C.c_int64), b.size, ms))
         return [ms[i] for i in range(b.size)]
+
+
+class Run:
+    """Step-wise batch: chains admitted once, advanced `sweeps` at a time."""
+
+    def __init__(self, ctx: Context, jobs: list[tuple[int, Job]]):
+        self.ctx = ctx
+        self.jobs = jobs
+        self._arr = (_capi.Job * len(jobs))(*[j.struct(gid) for gid, j in jobs])
+        self._h = C.c_void_p()
+        check(lib().rfmf_run_create(ctx.handle, self._arr, len(jobs), C.byref(self._h)))
+
+    def advance(self, sweeps: int, use_hip_graph: bool = False, grid_cap: int = 0) -> tuple[float, bool]:
+        tun = _capi.Tuning(0, int(grid_cap), int(bool(use_hip_graph)), 0, (C.c_int32 * 3)())
+        ms = C.c_float()
+        fin = C.c_int32()
+        check(lib().rfmf_run_advance(self._h, int(sweeps), C.byref(tun), C.byref(ms), C.byref(fin)))
+        return ms.value, bool(fin.value)
+
+    def collect(self) -> None:
+        check(lib().rfmf_run_collect(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            lib().rfmf_run_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class Server:

@@ -457,6 +457,90 @@ rfmf_status rfmf_batch_run(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs, 
     });
 }
 
+}  // extern "C"
+
+struct rfmf_run {
+    rfmf_ctx* ctx = nullptr;
+    std::vector<rfmf::JobSpec> specs;
+    std::vector<std::unique_ptr<rfmf::ResidentJob>> jobs;
+    std::vector<rfmf::ResidentJob*> ptrs;
+    std::unique_ptr<rfmf::Context::GraphedSlices> graph;
+    int64_t graphSlice = 0;
+    size_t graphActive = 0;
+
+    size_t activeChains() const {
+        size_t a = 0;
+        for (const auto* j : ptrs)
+            for (uint64_t s : j->stepsDone) a += s < j->totalSteps;
+        return a;
+    }
+};
+
+extern "C" {
+
+rfmf_status rfmf_run_create(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs, rfmf_run** out) {
+    return guarded([&] {
+        if (!ctx || !out || n_jobs <= 0 || !jobs) throw Error(Status::InvalidArg, "invalid run");
+        std::lock_guard<std::mutex> lk(ctx->ctx->mutex());
+        rfmf::Context& c = *ctx->ctx;
+        auto r = std::make_unique<rfmf_run>();
+        r->ctx = ctx;
+        for (int64_t i = 0; i < n_jobs; ++i) r->specs.push_back(specFromJob(jobs[i], c.graph(jobs[i].graph_id)));
+        for (const auto& s : r->specs) {
+            r->jobs.push_back(c.admit(s, c.stream()));
+            r->ptrs.push_back(r->jobs.back().get());
+        }
+        *out = r.release();
+    });
+}
+
+rfmf_status rfmf_run_advance(rfmf_run* run, int64_t sweeps, const rfmf_tuning* tuning, float* kernel_ms,
+                             int32_t* finished) {
+    return guarded([&] {
+        if (!run || sweeps <= 0) throw Error(Status::InvalidArg, "invalid run or sweep count");
+        std::lock_guard<std::mutex> lk(run->ctx->ctx->mutex());
+        rfmf::Context& c = *run->ctx->ctx;
+        const rfmf::LaunchKnobs k = knobsFrom(tuning);
+        float ms = 0.0f;
+        const size_t active = run->activeChains();
+        if (active > 0) {
+            if (k.hipGraph) {
+                if (!run->graph || run->graphSlice != sweeps || run->graphActive != active) {
+                    run->graph = c.captureSlices(run->ptrs, static_cast<uint64_t>(sweeps), k.gridCap);
+                    run->graphSlice = sweeps;
+                    run->graphActive = active;
+                }
+                ms = c.replaySlices(*run->graph, 1);
+            } else {
+                rfmf::LaunchKnobs s = k;
+                s.sliceSweeps = sweeps;
+                ms = c.runSlice(run->ptrs, s);
+            }
+        }
+        if (kernel_ms) *kernel_ms = ms;
+        if (finished) *finished = run->activeChains() == 0 ? 1 : 0;
+    });
+}
+
+rfmf_status rfmf_run_collect(rfmf_run* run) {
+    return guarded([&] {
+        if (!run) throw Error(Status::InvalidArg, "null run");
+        std::lock_guard<std::mutex> lk(run->ctx->ctx->mutex());
+        rfmf::Context& c = *run->ctx->ctx;
+        for (size_t i = 0; i < run->specs.size(); ++i) c.collect(*run->jobs[i], run->specs[i], c.stream());
+    });
+}
+
+void rfmf_run_destroy(rfmf_run* run) {
+    if (!run) return;
+    {
+        std::lock_guard<std::mutex> lk(run->ctx->ctx->mutex());
+        run->graph.reset();
+        run->jobs.clear();
+    }
+    delete run;
+}
+
 rfmf_status rfmf_batch_run_windows(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs,
                                    const int64_t* step_bounds, int64_t n_windows, float* window_ms) {
     return guarded([&] {

@@ -302,61 +302,119 @@ float Context::runSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
     return waitSlice(h);
 }
 
+std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<ResidentJob*>& jobs, int gridCap) {
+    // Group unfinished chains by kernel variant; items sorted by descending n
+    // so the persistent workgroups take the longest chains first.
+    std::vector<LaunchGroup> groups(6);
+    std::vector<int64_t> maxN(6, 0);
+    std::vector<int> maxP(6, 1);
+    auto gidx = [](int bits, bool mul24) { return (bits == 8 ? 0 : bits == 16 ? 1 : 2) * 2 + (mul24 ? 1 : 0); };
+    for (ResidentJob* j : jobs) {
+        const int gi = gidx(j->bits, j->mul24);
+        for (size_t c = 0; c < j->items.size(); ++c) {
+            if (j->stepsDone[c] >= j->totalSteps) continue;
+            groups[gi].items.push_back(j->items[c]);
+            maxN[gi] = std::max<int64_t>(maxN[gi], j->graph->n);
+            maxP[gi] = std::max(maxP[gi], j->maxP);
+        }
+    }
+    std::vector<LaunchGroup> out;
+    for (int gi = 0; gi < 6; ++gi) {
+        LaunchGroup& g = groups[gi];
+        if (g.items.empty()) continue;
+        std::stable_sort(g.items.begin(), g.items.end(),
+                         [](const dev::ChainItem& a, const dev::ChainItem& b) { return a.n > b.n; });
+        g.bits = (gi / 2 == 0) ? 8 : (gi / 2 == 1) ? 16 : 32;
+        g.mul24 = (gi % 2) == 1;
+        g.plan = dev::makeLdsPlan(maxN[gi], g.bits, maxP[gi]);
+        if (static_cast<size_t>(g.plan.total) > props_.sharedMemPerBlock)
+            throw Error(Status::Unsupported, "combined launch needs " + std::to_string(g.plan.total) +
+                                                 " B of LDS per workgroup");
+        const int occ = std::max(1, occupancy(g.bits, g.mul24, g.plan.total));
+        int grid = occ * props_.multiProcessorCount;
+        if (gridCap > 0) grid = std::min(grid, gridCap);
+        g.grid = static_cast<int>(std::min<int64_t>(grid, static_cast<int64_t>(g.items.size())));
+        out.push_back(std::move(g));
+    }
+    return out;
+}
+
 Context::SliceHandle Context::launchSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs) {
     hipCheck(hipSetDevice(device_), "hipSetDevice");
     SliceHandle h;
     h.jobs = jobs;
-    // Group unfinished chains by kernel variant; items sorted by descending n
-    // so the persistent workgroups take the longest chains first.
-    struct Group {
-        std::vector<dev::ChainItem> items;
-        int64_t maxN = 0;
-        int maxP = 1;
-    };
-    std::vector<Group> groups(6);
-    auto gidx = [](int bits, bool mul24) { return (bits == 8 ? 0 : bits == 16 ? 1 : 2) * 2 + (mul24 ? 1 : 0); };
-    for (ResidentJob* j : jobs) {
-        Group& g = groups[gidx(j->bits, j->mul24)];
-        for (size_t c = 0; c < j->items.size(); ++c) {
-            if (j->stepsDone[c] >= j->totalSteps) continue;
-            g.items.push_back(j->items[c]);
-            g.maxN = std::max<int64_t>(g.maxN, j->graph->n);
-            g.maxP = std::max(g.maxP, j->maxP);
-        }
-    }
+    std::vector<LaunchGroup> groups = planGroups(jobs, knobs.gridCap);
     const uint64_t slice = knobs.sliceSweeps > 0 ? static_cast<uint64_t>(knobs.sliceSweeps) : ~uint64_t{0};
     h.slice = slice;
-    std::vector<DevBuf>& itemBufs = h.itemBufs;
     hipCheck(hipEventCreate(&h.ev0), "hipEventCreate");
     hipCheck(hipEventCreate(&h.ev1), "hipEventCreate");
     hipCheck(hipEventRecord(h.ev0, stream_), "hipEventRecord");
     int slot = 0;
-    for (int gi = 0; gi < 6; ++gi) {
-        Group& g = groups[gi];
-        if (g.items.empty()) continue;
-        std::stable_sort(g.items.begin(), g.items.end(),
-                         [](const dev::ChainItem& a, const dev::ChainItem& b) { return a.n > b.n; });
-        const int bits = (gi / 2 == 0) ? 8 : (gi / 2 == 1) ? 16 : 32;
-        const bool mul24 = (gi % 2) == 1;
-        const dev::LdsPlan plan = dev::makeLdsPlan(g.maxN, bits, g.maxP);
-        if (static_cast<size_t>(plan.total) > props_.sharedMemPerBlock)
-            throw Error(Status::Unsupported, "combined launch needs " + std::to_string(plan.total) +
-                                                 " B of LDS per workgroup");
-        const int occ = std::max(1, occupancy(bits, mul24, plan.total));
-        int grid = occ * props_.multiProcessorCount;
-        if (knobs.gridCap > 0) grid = std::min(grid, knobs.gridCap);
-        grid = static_cast<int>(std::min<int64_t>(grid, static_cast<int64_t>(g.items.size())));
-        itemBufs.emplace_back(g.items.size() * sizeof(dev::ChainItem));
-        hipCheck(hipMemcpyAsync(itemBufs.back().get(), g.items.data(), itemBufs.back().bytes(),
+    for (LaunchGroup& g : groups) {
+        h.itemBufs.emplace_back(g.items.size() * sizeof(dev::ChainItem));
+        hipCheck(hipMemcpyAsync(h.itemBufs.back().get(), g.items.data(), h.itemBufs.back().bytes(),
                                 hipMemcpyHostToDevice, stream_), "upload items");
-        hipCheck(launch::anneal(bits, mul24, itemBufs.back().as<dev::ChainItem>(),
-                                static_cast<int>(g.items.size()), sched_.as<uint32_t>() + 2 * slot, plan,
-                                grid, slice, stream_), "annealSlice launch");
+        hipCheck(launch::anneal(g.bits, g.mul24, h.itemBufs.back().as<dev::ChainItem>(),
+                                static_cast<int>(g.items.size()), sched_.as<uint32_t>() + 2 * slot, g.plan,
+                                g.grid, slice, stream_), "annealSlice launch");
         ++slot;
         h.empty = false;
     }
     hipCheck(hipEventRecord(h.ev1, stream_), "hipEventRecord");
     return h;
+}
+
+Context::GraphedSlices::~GraphedSlices() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+}
+
+std::unique_ptr<Context::GraphedSlices> Context::captureSlices(const std::vector<ResidentJob*>& jobs,
+                                                               uint64_t sliceSweeps, int gridCap) {
+    hipCheck(hipSetDevice(device_), "hipSetDevice");
+    auto g = std::make_unique<GraphedSlices>();
+    g->jobs = jobs;
+    g->slice = sliceSweeps;
+    std::vector<LaunchGroup> groups = planGroups(jobs, gridCap);
+    // Item arrays go up before the capture; the graph holds kernel nodes only.
+    for (LaunchGroup& lg : groups) {
+        g->itemBufs.emplace_back(lg.items.size() * sizeof(dev::ChainItem));
+        hipCheck(hipMemcpy(g->itemBufs.back().get(), lg.items.data(), g->itemBufs.back().bytes(),
+                           hipMemcpyHostToDevice), "upload items");
+        (void)occupancy(lg.bits, lg.mul24, lg.plan.total);   // sets the LDS attribute outside the capture
+    }
+    hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    int slot = 0;
+    hipError_t err = hipSuccess;
+    for (size_t i = 0; i < groups.size() && err == hipSuccess; ++i, ++slot)
+        err = launch::anneal(groups[i].bits, groups[i].mul24, g->itemBufs[i].as<dev::ChainItem>(),
+                             static_cast<int>(groups[i].items.size()), sched_.as<uint32_t>() + 2 * slot,
+                             groups[i].plan, groups[i].grid, sliceSweeps, stream_);
+    const hipError_t endErr = hipStreamEndCapture(stream_, &g->graph);
+    hipCheck(err, "annealSlice capture");
+    hipCheck(endErr, "hipStreamEndCapture");
+    hipCheck(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    return g;
+}
+
+float Context::replaySlices(GraphedSlices& g, int times) {
+    hipCheck(hipSetDevice(device_), "hipSetDevice");
+    hipEvent_t e0, e1;
+    hipCheck(hipEventCreate(&e0), "hipEventCreate");
+    hipCheck(hipEventCreate(&e1), "hipEventCreate");
+    hipCheck(hipEventRecord(e0, stream_), "hipEventRecord");
+    for (int i = 0; i < times; ++i) hipCheck(hipGraphLaunch(g.exec, stream_), "hipGraphLaunch");
+    hipCheck(hipEventRecord(e1, stream_), "hipEventRecord");
+    hipCheck(hipEventSynchronize(e1), "annealSlice graph");
+    float ms = 0.0f;
+    hipCheck(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const uint64_t adv = g.slice * static_cast<uint64_t>(times);
+    for (ResidentJob* j : g.jobs)
+        for (auto& s : j->stepsDone) s = (j->totalSteps - s > adv) ? s + adv : j->totalSteps;
+    return ms;
 }
 
 float Context::waitSlice(SliceHandle& h) {
@@ -377,6 +435,19 @@ float Context::waitSlice(SliceHandle& h) {
 
 float Context::runToCompletion(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs) {
     float total = 0.0f;
+    if (knobs.hipGraph && knobs.sliceSweeps > 0) {
+        // Every chain starts together here, so ceil(longest / slice) replays
+        // of one captured slice finish them all.
+        uint64_t longest = 0;
+        for (const ResidentJob* j : jobs)
+            for (uint64_t s : j->stepsDone) longest = std::max(longest, j->totalSteps - s);
+        if (longest > 0) {
+            auto g = captureSlices(jobs, static_cast<uint64_t>(knobs.sliceSweeps), knobs.gridCap);
+            const uint64_t reps = (longest + knobs.sliceSweeps - 1) / knobs.sliceSweeps;
+            total += replaySlices(*g, static_cast<int>(reps));
+        }
+        return total;
+    }
     for (;;) {
         bool pending = false;
         for (const ResidentJob* j : jobs)

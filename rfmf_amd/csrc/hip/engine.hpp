@@ -129,6 +129,21 @@ public:
     float waitSlice(SliceHandle& h);
     // Runs jobs to completion (slices repeated) and returns kernel ms.
     float runToCompletion(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs);
+
+    // A fixed set of chains whose slice launches are captured once into a
+    // hipGraph (item arrays uploaded once) and replayed per slice.
+    struct GraphedSlices {
+        std::vector<DevBuf> itemBufs;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        std::vector<ResidentJob*> jobs;
+        uint64_t slice = 0;
+        ~GraphedSlices();
+    };
+    std::unique_ptr<GraphedSlices> captureSlices(const std::vector<ResidentJob*>& jobs, uint64_t sliceSweeps,
+                                                 int gridCap);
+    // Replays the captured slice `times` times; returns kernel-side ms.
+    float replaySlices(GraphedSlices& g, int times);
     // Copies a finished job's results to the spec's outputs.
     void collect(const ResidentJob& job, const JobSpec& spec, hipStream_t stream);
 
@@ -138,6 +153,15 @@ public:
     std::mutex& mutex() { return mu_; }
 
 private:
+    struct LaunchGroup {
+        std::vector<dev::ChainItem> items;
+        int bits = 8;
+        bool mul24 = false;
+        dev::LdsPlan plan{};
+        int grid = 1;
+    };
+    std::vector<LaunchGroup> planGroups(const std::vector<ResidentJob*>& jobs, int gridCap);
+
     int device_;
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;

@@ -1,0 +1,156 @@
+"""Batched multi-graph launches, hipGraph replay, slicing, chain isolation,
+the step-wise run API and the serving loop: every chain's result must be
+byte-identical to its one-shot run (and to the oracle)."""
+import time
+
+import numpy as np
+import pytest
+
+import rfmf_amd as R
+from rfmf_amd import workloads as W
+from conftest import run_oracle_chains
+
+pytestmark = pytest.mark.gpu
+
+LADDER = [128, 64, 32, 16, 8, 4, 2, 1]
+
+
+def make_job(csr, seeds, phases, iters, linear=True, t0=None):
+    init = np.stack([R.generate_random_spins(s, csr.n) for s in seeds])
+    solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+    if linear:
+        beta = R.build_linear_beta_table(0.1, 3.0, sum(iters), len(seeds))
+    else:
+        beta = R.build_beta_run_tables(t0, sum(iters))
+    return R.Job(csr, init, solver, beta, phases, iters)
+
+
+def one_shot(job):
+    return R.anneal_gpu_tables(job.csr, R.max_weighted_degree(job.csr), job.init, job.beta, job.phases,
+                               job.iters, job.seeds)
+
+
+def same(job, res, label):
+    assert np.array_equal(job.end_spins, res.end_spins), f"{label}: final spins differ"
+    assert np.array_equal(job.best_spins, res.best_spins), f"{label}: best spins differ"
+    assert np.array_equal(job.best_vals, res.kernel_best_vals), f"{label}: best cuts differ"
+    assert np.array_equal(job.final_vals, res.kernel_final_vals), f"{label}: final cuts differ"
+
+
+@pytest.fixture(scope="module")
+def graphs(gset):
+    return {
+        "G11": gset("G11"),
+        "ER800": W.erdos_renyi(800, 0.06, seed=3),      # 16-bit words
+        "G70": gset("G70"),
+        "deg": W.degree_random(1500, 2, 6, seed=4),
+        "torus": W.torus(30, 60, seed=5),
+    }
+
+
+def _mixed_jobs(graphs):
+    return [
+        ("G11", make_job(graphs["G11"], [1, 2, 3], [8, 4, 2, 1], [60] * 4)),
+        ("ER800", make_job(graphs["ER800"], [4, 5], LADDER, [25] * 8)),
+        ("G70", make_job(graphs["G70"], [6], [8, 2, 1], [50, 50, 50], linear=False, t0=[105.0])),
+        ("deg", make_job(graphs["deg"], [7, 8, 9, 10], [3, 64, 1], [40, 40, 40])),
+        ("torus", make_job(graphs["torus"], [11, 12], LADDER, [30] * 8)),
+    ]
+
+
+def test_batch_mixed_graphs_equal_one_shot(graphs):
+    print("[batch] 5 graphs, mixed word widths and schedules, one batch", flush=True)
+    jobs = _mixed_jobs(graphs)
+    with R.Context(0) as ctx:
+        ids = {k: ctx.upload_graph(g) for k, g in graphs.items()}
+        ctx.batch_run([(ids[k], j) for k, j in jobs])
+    for k, j in jobs:
+        same(j, one_shot(j), f"batch {k}")
+    k, j = jobs[3]
+    end, best, cv, bv = run_oracle_chains(j.csr, j.init, j.seeds, j.beta, j.phases, j.iters)
+    assert np.array_equal(j.end_spins, end) and np.array_equal(j.best_spins, best)
+
+
+@pytest.mark.parametrize("slice_sweeps,use_graph", [(1, False), (37, False), (50, True), (999, True)])
+def test_slicing_and_graph_replay_invariance(graphs, slice_sweeps, use_graph):
+    print(f"[batch] slice={slice_sweeps} hipGraph={use_graph}", flush=True)
+    jobs = _mixed_jobs(graphs)
+    with R.Context(0) as ctx:
+        ids = {k: ctx.upload_graph(g) for k, g in graphs.items()}
+        ctx.batch_run([(ids[k], j) for k, j in jobs], slice_sweeps=slice_sweeps, use_hip_graph=use_graph)
+    for k, j in jobs:
+        same(j, one_shot(j), f"slice {slice_sweeps} {k}")
+
+
+def test_chain_isolation_beyond_resident_capacity(gset):
+    # 3000 chains of an 800-vertex graph exceed the resident workgroups of
+    # 256 CUs, so the persistent workgroups take several chains each.
+    csr = gset("G11")
+    seeds = list(range(501, 3501))
+    print(f"[isolation] {len(seeds)} chains of G11", flush=True)
+    job = make_job(csr, seeds, [8, 1], [60, 40])
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ctx.batch_run([(gid, job)])
+    for c in (0, 1, 1500, 2999):
+        alone = make_job(csr, [seeds[c]], [8, 1], [60, 40])
+        r = one_shot(alone)
+        assert np.array_equal(job.end_spins[c], r.end_spins[0]), f"chain {c} differs from its C=1 run"
+        assert np.array_equal(job.best_spins[c], r.best_spins[0])
+        assert job.best_vals[c] == r.kernel_best_vals[0]
+
+
+def test_run_api_prefix_matches_oracle(graphs):
+    csr = graphs["torus"]
+    phases, iters = LADDER, [40] * 8
+    job = make_job(csr, [21, 22, 23], phases, iters)
+    print("[run] advance 100 + collect, then to completion", flush=True)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        with R.Run(ctx, [(gid, job)]) as run:
+            _, fin = run.advance(100)
+            assert not fin
+            run.collect()
+            mid_end, mid_best, mid_bv = job.end_spins.copy(), job.best_spins.copy(), job.best_vals.copy()
+            done = False
+            while not done:
+                _, done = run.advance(70, use_hip_graph=True)
+            run.collect()
+    # prefix: the first 100 sweeps are phases 128, 64 (40 each) and 20 of P=32
+    end, best, cv, bv = run_oracle_chains(csr, job.init, job.seeds, job.beta, [128, 64, 32], [40, 40, 20])
+    assert np.array_equal(mid_end, end) and np.array_equal(mid_best, best) and np.array_equal(mid_bv, bv)
+    end, best, cv, bv = run_oracle_chains(csr, job.init, job.seeds, job.beta, phases, iters)
+    assert np.array_equal(job.end_spins, end) and np.array_equal(job.best_spins, best)
+    assert np.array_equal(job.final_vals, cv) and np.array_equal(job.best_vals, bv)
+
+
+def test_server_results_equal_one_shot(graphs):
+    reqs = [
+        ("G11", make_job(graphs["G11"], [31, 32], [8, 4, 2, 1], [80] * 4)),
+        ("torus", make_job(graphs["torus"], [33], LADDER, [50] * 8)),
+        ("ER800", make_job(graphs["ER800"], [34, 35, 36], LADDER, [20] * 8)),
+        ("deg", make_job(graphs["deg"], [37], [3, 64, 1], [100, 100, 100])),
+        ("G70", make_job(graphs["G70"], [38, 39], [8, 2, 1], [60, 60, 60], linear=False, t0=[90.0, 100.0])),
+        ("G11b", make_job(graphs["G11"], [40], [16, 1], [300, 300])),
+    ]
+    fps = [R.footprint_bytes(j.csr.n, j.csr.m2, j.chains, j.beta.run_end.size, len(j.phases)) for _, j in reqs]
+    budget = sum(sorted(fps)[-2:])   # at most about two requests resident: queueing is exercised
+    print(f"[serve] {len(reqs)} requests, budget {budget} B", flush=True)
+    with R.Context(0) as ctx:
+        with R.Server(ctx, budget, slice_sweeps=64) as srv:
+            ids = []
+            for k, j in reqs:
+                ids.append(srv.submit(j))
+                time.sleep(0.002)
+            big = make_job(graphs["torus"], list(range(100, 400)), LADDER, [10] * 8)
+            with pytest.raises(R.RfmfError) as ei:
+                srv.submit(big)
+            assert ei.value.status == 10
+            for rid in ids:
+                srv.wait(rid, 120.0)
+            lat = [srv.poll(rid) for rid in ids]
+            st = srv.stats()
+    assert all(d for d, *_ in lat)
+    assert st["completed"] == len(reqs) and st["rejected"] == 1
+    for (k, j) in reqs:
+        same(j, one_shot(j), f"served {k}")

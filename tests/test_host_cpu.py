@@ -1,0 +1,243 @@
+"""Host side of the product library on the CPU: the C ABI's exports, the load
+path (Gset reader, dedup, degree-zero removal, dense CSR), beta tables, the
+int32 gate, initial spins and cut recompute against the oracle and the
+reference's own host code; error codes of the drop-in entry point; the CLI's
+argument handling; synthetic workload digests."""
+import ctypes as C
+import json
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import rfmf_amd as R
+from rfmf_amd import _capi, workloads as W
+from conftest import GOLDEN, GSET, ROOT
+from oracle import oracle as O
+
+REF = json.loads((GOLDEN / "ref_vectors.json").read_text())
+
+
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "rfmf.h").read_text()
+    names = set(re.findall(r"\b(rfmf_[a-z0-9_]+)\s*\(", header))
+    lib = C.CDLL(str(_capi.LIB_PATH))
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert names <= set(_capi.SIGNATURES), sorted(names - set(_capi.SIGNATURES))
+    assert R.lib().rfmf_version().decode().startswith("rfmf")
+
+
+@pytest.mark.parametrize("inst", ["G1", "G11", "G61", "G62", "G70", "G81"])
+def test_csr_product_equals_oracle(inst):
+    csr, st = R.read_instance_from_file(str(GSET / inst))
+    off, to, w, hn, hm = O.read_gset(GSET / inst)
+    assert np.array_equal(csr.offsets, off) and np.array_equal(csr.edge_to, to)
+    assert np.array_equal(csr.edge_weight, w)
+    assert st.num_vertices == hn and st.num_edges == hm and st.scale == 1.0
+    assert R.max_weighted_degree(csr) == st.max_abs_weighted_degree == O.max_weighted_degree(off, to, w)
+    assert np.all(np.diff(csr.original_ids) > 0)
+
+
+def test_graph_from_edges_first_weight_wins_and_dense_ranks():
+    u = [10, 30, 10, 30, 7]
+    v = [30, 10, 20, 20, 10]
+    w = [5, -9, 2, 3, 1]          # (30, 10) repeats (10, 30): first weight 5 wins
+    csr = R.graph_from_edges(u, v, w)
+    assert csr.original_ids.tolist() == [7, 10, 20, 30]
+    assert csr.offsets.tolist() == [0, 1, 4, 6, 8]
+    assert csr.edge_to.tolist() == [1, 0, 2, 3, 1, 3, 1, 2]
+    assert csr.edge_weight.tolist() == [1, 1, 2, 5, 2, 3, 5, 3]
+    off, to, ww, ids = O.build_csr(u, v, w)
+    assert np.array_equal(off, csr.offsets) and np.array_equal(to, csr.edge_to)
+    assert np.array_equal(ww, csr.edge_weight) and ids.tolist() == [7, 10, 20, 30]
+
+
+def _write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_bytes(text.encode())
+    return p
+
+
+READER_CASES = {
+    "crlf_comments": "# header comment\r\n\r\n4 4\r\n1 2 1\r\n  # inner\r\n2 3 -1\r\n3 4 1\r\n4 1 2\r\n",
+    "float_weights": "3 3\n1 2 0.5\n2 3 0.25\n3 1 -1.75\n",
+    "zero_dropped": "4 4\n1 2 0\n2 3 1\n3 4 0.0\n1 4 3\n",
+    "duplicates": "3 4\n1 2 4\n2 1 7\n2 3 1\n3 2 9\n",
+    "sparse_ids": "100 3\n5 90 1\n90 17 1\n17 5 -1\n",
+    "tiny_floats": "2 1\n1 2 0.000123\n",
+}
+
+
+@pytest.mark.parametrize("case", sorted(READER_CASES))
+def test_reader_matches_reference_loader(tmp_path, case):
+    p = _write(tmp_path, case, READER_CASES[case])
+    csr, st = R.read_instance_from_file(str(p))
+    if not O.ref_driver_available():
+        pytest.skip("reference host build not present")
+    lines = O.ref_driver("csr", p).splitlines()
+    n, m2, B, scale, zero, maxabs = lines[0].split()
+    assert csr.n == int(n) and csr.m2 == int(m2)
+    assert st.scale == float(scale) and st.zero_dropped == int(zero)
+    assert st.max_abs_weighted_degree == int(maxabs) == R.max_weighted_degree(csr) == int(B)
+    assert csr.offsets.tolist() == [int(x) for x in lines[1].split()]
+    assert csr.edge_to.tolist() == [int(x) for x in lines[2].split()] if lines[2] else csr.m2 == 0
+    assert csr.edge_weight.tolist() == [int(x) for x in lines[3].split()] if lines[3] else csr.m2 == 0
+
+
+def test_reader_scale_values(tmp_path):
+    # maxB = 2.0 (vertex 1: 0.25 + 1.75), maxW = 1.75, maxDeg = 2: the largest
+    # 10^k with 2*10^k + 1 <= INT32_MAX/4 is 10^8.
+    _, st = R.read_instance_from_file(str(_write(tmp_path, "f", READER_CASES["float_weights"])))
+    assert st.float_weights and st.scale == 1e8
+    csr, st = R.read_instance_from_file(str(_write(tmp_path, "z", READER_CASES["zero_dropped"])))
+    assert st.zero_dropped == 2 and csr.n == 4 and csr.m2 == 4
+
+
+def test_reader_errors(tmp_path):
+    with pytest.raises(R.RfmfError) as e:
+        R.read_instance_from_file(str(_write(tmp_path, "bad", "3 2\n1 2 1\n")))
+    assert e.value.status == 8
+    with pytest.raises(R.RfmfError):
+        R.read_instance_from_file(str(tmp_path / "missing"))
+
+
+def test_spins_product_equals_oracle_and_reference():
+    for rec in REF["spins"]:
+        s = R.generate_random_spins(rec["seed"], rec["n"])
+        assert "".join("1" if x > 0 else "0" for x in s) == rec["bits"]
+        assert np.array_equal(s, O.initial_spins(rec["seed"], rec["n"]))
+
+
+@pytest.mark.parametrize("t0s,total", [([35.0, 50.0, 35.0], 200_000), ([0.5], 30_000), ([400.0, 1.0], 123_457),
+                                       ([2.0], 0), ([7.0], 1)])
+def test_log_tables_product_equals_oracle_bruteforce(t0s, total):
+    t = R.build_beta_run_tables(t0s, total)
+    for c, t0 in enumerate(t0s):
+        e, v = O.beta_runs_log(t0, total, cap=total + 2)
+        te, tv = t.table(c)
+        assert np.array_equal(te, e) and np.array_equal(tv, v)
+
+
+def test_log_tables_equal_reference_digests():
+    import hashlib
+    for rec in REF["beta_tables"]:
+        t = R.build_beta_run_tables(rec["t0s"], rec["total"])
+        assert t.run_end.size == rec["n_runs"] and t.n_tables == rec["n_tables"]
+        assert t.chain_run_off.tolist() == rec["chain_run_off"]
+        assert hashlib.sha256(t.run_end.tobytes()).hexdigest() == rec["run_end_sha"]
+        assert hashlib.sha256(t.run_val.tobytes()).hexdigest() == rec["run_val_sha"]
+
+
+def test_linear_table_product_equals_oracle():
+    for b0, b1, T in [(0.1, 3.0, 10_000), (0.1, 3.0, 2_000_000), (1.0, 1.0, 50), (2.0, 0.5, 777)]:
+        t = R.build_linear_beta_table(b0, b1, T, 3)
+        e, v = O.beta_runs_linear(int(round(b0 * 2**20)), int(round(b1 * 2**20)), T)
+        assert np.array_equal(t.run_end, e) and np.array_equal(t.run_val, v)
+
+
+def test_gate_and_beta_fixed_product_equals_oracle():
+    rng = np.random.default_rng(5)
+    for rec in REF["gate"]:
+        assert R.anneal_score_safe_int32(rec["total"], rec["t0"], rec["B"]) == bool(rec["safe"])
+    for _ in range(500):
+        T = int(10 ** rng.uniform(0, 12))
+        t0 = float(10 ** rng.uniform(-3, 3))
+        B = int(10 ** rng.uniform(0, 9))
+        assert R.anneal_score_safe_int32(T, t0, B) == O.score_safe_int32(T, t0, B)
+        s = int(rng.integers(1, 2**40))
+        assert R.beta_fixed_for_step(s, t0) == O.beta_fixed(s, t0)
+
+
+def test_cut_product_equals_oracle():
+    csr, _ = R.read_instance_from_file(str(GSET / "G61"))
+    rng = np.random.default_rng(9)
+    for _ in range(5):
+        s = rng.choice([-1, 1], csr.n).astype(np.int8)
+        assert R.cut_value_csr(csr, s) == O.cut_value(csr.offsets, csr.edge_to, csr.edge_weight, s)
+
+
+def test_total_steps_cap():
+    assert R.anneal_total_steps([1, 2, 3]) == 6
+    with pytest.raises(R.RfmfError) as e:
+        R.anneal_total_steps([10**12, 1])
+    assert e.value.status == 7
+
+
+# ---------------------------------------------------------------- drop-in error behaviour (no device needed)
+
+def _g11():
+    return R.read_instance_from_file(str(GSET / "G11"))[0]
+
+
+def test_anneal_rejects_bad_inputs_before_device_work():
+    csr = _g11()
+    init = np.ones((2, csr.n), np.int8)
+    seeds = [1, 2]
+    with pytest.raises(R.RfmfError) as e:
+        R.anneal_gpu(csr, 4, init, [1.0, 0.0], [8], [10], seeds)
+    assert e.value.status == 1                         # t0 <= 0
+    with pytest.raises(R.RfmfError) as e:
+        R.anneal_gpu(csr, 4, init, [1.0, 2.0], [8, 1], [10**12, 1], seeds)
+    assert e.value.status == 7                         # total-sweep cap
+    with pytest.raises(R.RfmfError) as e:
+        R.anneal_gpu(csr, 4, init, [0.001, 2.0], [8], [10**9], seeds)
+    assert e.value.status in (2, 6)                    # int64 path / run cap: both before any device call
+
+
+def test_anneal_passthrough_without_phases():
+    csr = _g11()
+    init = np.stack([R.generate_random_spins(s, csr.n) for s in (3, 4)])
+    r = R.anneal_gpu(csr, 4, init, [1.0, 2.0], [], [], [5, 6])
+    assert np.array_equal(r.end_spins, init) and np.array_equal(r.best_spins, init)
+    assert r.kernel_ms == 0.0
+    assert [int(x) for x in r.kernel_best_vals] == [R.cut_value_csr(csr, init[c]) for c in range(2)]
+
+
+def test_context_without_device_reports_no_device():
+    import os
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is visible")
+    with pytest.raises(R.RfmfError) as e:
+        R.Context(0)
+    assert e.value.status == 4
+
+
+# ---------------------------------------------------------------- CLI
+
+CLI = ROOT / "rfmf_amd" / "bin" / "rfmf_main"
+
+
+@pytest.mark.parametrize("args,needle", [
+    ([], "Usage"),
+    (["x", "--seed", "1"], "one chain per (seed, t0) pair"),
+    (["x", "--seeds", "1,2", "--t0s", "1", "--phases", "1", "--iters", "1"], "same number of entries"),
+    (["x", "--seeds", "1", "--t0s", "0", "--phases", "1", "--iters", "1"], "must be > 0"),
+    (["x", "--seeds", "1", "--t0s", "1", "--phases", "1,2", "--iters", "1"], "same number of elements"),
+    (["x", "--bogus", "1"], "unknown option"),
+])
+def test_cli_argument_errors(args, needle):
+    r = subprocess.run([str(CLI), *args], capture_output=True, text=True)
+    assert r.returncode == 1 and needle in r.stderr
+
+
+# ---------------------------------------------------------------- workloads
+
+def test_workload_digests_are_stable():
+    assert W.csr_digest(W.torus(100, 200, seed=0)) == "52f133f8b6e30961"
+    er = W.erdos_renyi(800, 0.06, seed=0)
+    assert er.n == 800 and abs(er.m2 / 2 - 0.06 * 800 * 799 / 2) < 600
+    sp = W.sparse_random(10000, 10000, False, seed=0)
+    assert sp.m2 == 20000 and 8000 < sp.n < 9300
+    pm = W.sparse_random(7000, 17150, True, seed=0)
+    assert pm.m2 == 34300 and set(np.unique(pm.edge_weight)) == {-1, 1}
+    dg = W.degree_random(2000, 2, 6, seed=0)
+    assert dg.degrees().max() <= 6
+
+
+def test_footprint_grows_with_chains():
+    a = R.footprint_bytes(20000, 80000, 64, 1000, 8)
+    b = R.footprint_bytes(20000, 80000, 128, 1000, 8)
+    assert b > a > 20000 * 64

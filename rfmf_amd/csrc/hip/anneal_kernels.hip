@@ -6,28 +6,35 @@
 // the pre-sweep state (score = Gumbel + beta * spin*S, Philox4x32-10 keyed by
 // the chain's solver seed, counter (sweep, v>>3)) and keeps the per-group
 // argmax of group g = v mod P (ties to the smaller id); FLIP flips every
-// group's winner, adds the winner-winner coupling corr[] so the cut advances
-// exactly as sequential group-order flips would, and scatters the neighbour
-// field updates.
+// group's winner and advances the cut exactly as sequential group-order
+// flips would (the winner-winner coupling corr[]), then updates the fields.
 //
 // The execution is MI355X-native and differs from the reference throughout:
 //   * one 256-thread workgroup (4 wave64) anneals one chain at a time, and the
 //     chain's whole state lives in LDS for the length of a slice: one packed
-//     word per vertex (spin bit | biased S, 8/16/32 bits chosen from the
-//     graph's max weighted degree) plus a best-spin bitmask;
+//     word per vertex holding the spin bit and delta = spin*S (biased to stay
+//     non-negative; 8/16/32 bits chosen from the graph's max weighted degree)
+//     plus a best-spin bitmask;
+//   * scoring is one v_mad_i32_i24 per vertex: the bias is folded into a
+//     per-sweep copy of the Gumbel LUT (LUT - bias*beta, wrapping mod 2^32;
+//     the final score is exact because it fits int32 under the gate);
 //   * the per-group argmax is kept in registers (each lane's 8 vertex slots
 //     map to fixed groups when P divides 8*threads), folded across lanes with
 //     a butterfly and across the 4 waves through LDS: no atomics on the hot
 //     path;
 //   * the winner map for corr[] is implicit: vertex x can only win group
-//     x mod P, so "x is an earlier winner" is pick[x mod P] == x with
-//     x mod P < g, and no n-sized winnerGroup array exists;
-//   * neighbour-field updates are 32-bit LDS atomics on the packed words;
+//     x mod P, so "x is a winner" is pick[x mod P] == x and no n-sized
+//     winnerGroup array exists;
+//   * FLIP updates non-winner neighbours in delta space with 32-bit LDS
+//     atomics on the packed words, and each winner's own word once, with its
+//     flip and every adjacent winner's coupling summed first;
 //   * the kernel is persistent over a work list of (graph, chain) items and
 //     runs each item for a slice of sweeps, saving the state to HBM at the end,
 //     so the same kernel serves single-graph runs, multi-graph batches and the
 //     serving loop.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "anneal_device.hpp"
 #include "anneal_launch.hpp"
@@ -47,6 +54,11 @@ constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
 constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
 constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
 
+// a ^ b ^ c in one v_bitop3_b32 (truth table 0x96).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 // Philox4x32-10 (Salmon et al. 2011): 10 rounds, key bumped between rounds.
 // The step words and key are wave-uniform, so the compiler keeps the first
 // round's uniform products on the scalar unit.
@@ -56,8 +68,8 @@ __device__ __forceinline__ void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = static_cast<uint64_t>(kPhiloxM0) * c0;
         const uint64_t p1 = static_cast<uint64_t>(kPhiloxM1) * c2;
-        const uint32_t n0 = static_cast<uint32_t>(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = static_cast<uint32_t>(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = xor3(static_cast<uint32_t>(p1 >> 32), c1, k0);
+        const uint32_t n2 = xor3(static_cast<uint32_t>(p0 >> 32), c3, k1);
         c1 = static_cast<uint32_t>(p1);
         c3 = static_cast<uint32_t>(p0);
         c0 = n0;
@@ -93,6 +105,15 @@ template <int kBits> struct Block8 {
             return (d[j / per] >> (kBits * (j % per))) & ((1u << kBits) - 1u);
         }
     }
+    // Biased delta field of word j.
+    __device__ __forceinline__ uint32_t field(int j) const {
+        if constexpr (kBits == 32) {
+            return d[j] & WordFmt<32>::kField;
+        } else {
+            constexpr int per = 32 / kBits;
+            return (d[j / per] >> (kBits * (j % per))) & WordFmt<kBits>::kField;
+        }
+    }
     // Spin bits of the 8 words as one byte (bit j = vertex 8b+j is +1).
     __device__ __forceinline__ uint32_t spinByte() const {
         uint32_t m = 0;
@@ -112,27 +133,23 @@ __device__ __forceinline__ uint32_t ldsWord(const uint32_t* wl, int x) {
     }
 }
 
-// delta(x) = spin(x) * S(x) from a packed word.
+// Adds d (mod 2^kBits, applied without carry or borrow because the word's
+// old and new values are both valid words) to word x.
 template <int kBits>
-__device__ __forceinline__ int32_t wordDelta(uint32_t w) {
-    using F = WordFmt<kBits>;
-    const int32_t S = static_cast<int32_t>(w & F::kField) - F::kBias;
-    return (w & F::kSpin) ? S : -S;
-}
-
-template <int kBits>
-__device__ __forceinline__ void ldsAddField(uint32_t* wl, int x, int32_t d) {
+__device__ __forceinline__ void ldsAddWord(uint32_t* wl, int x, uint32_t d) {
     constexpr int per = 32 / kBits;
     const int sh = kBits * (x % per);
-    atomicAdd(&wl[x / per], static_cast<uint32_t>(d) << sh);
+    atomicAdd(&wl[x / per], d << sh);
 }
 
-template <int kBits>
-__device__ __forceinline__ void ldsFlipSpin(uint32_t* wl, int x) {
-    constexpr int per = 32 / kBits;
-    const int sh = kBits * (x % per);
-    atomicXor(&wl[x / per], WordFmt<kBits>::kSpin << sh);
+__device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint32_t uni(uint32_t x) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int32_t>(x)));
 }
+__device__ __forceinline__ uint64_t uni(uint64_t x) {
+    return (static_cast<uint64_t>(uni(static_cast<uint32_t>(x >> 32))) << 32) | uni(static_cast<uint32_t>(x));
+}
+__device__ __forceinline__ int64_t uni(int64_t x) { return static_cast<int64_t>(uni(static_cast<uint64_t>(x))); }
 
 __device__ __forceinline__ uint64_t packKey(int32_t score, int32_t v) {
     if (v < 0) return 0ull;
@@ -158,6 +175,17 @@ __device__ __forceinline__ int64_t waveSum64(int64_t x) {
     return x;
 }
 
+// score = Gumbel + beta * delta, computed as beta * field + (LUT - bias*beta)
+// + fine offset, all mod 2^32 (the field is delta + bias).
+template <bool kMul24>
+__device__ __forceinline__ int32_t scoreOf(int32_t beta, uint32_t field, uint32_t lutBiased, uint32_t fine) {
+    const uint32_t base = lutBiased + fine;
+    if constexpr (kMul24)
+        return static_cast<int32_t>(static_cast<uint32_t>(__mul24(beta, static_cast<int32_t>(field))) + base);
+    else
+        return static_cast<int32_t>(static_cast<uint32_t>(beta) * field + base);
+}
+
 template <int kBits, bool kMul24, int NT>
 __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ items,
                                                   int nItems, uint32_t* __restrict__ sched,
@@ -172,7 +200,8 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
     int32_t* pk = reinterpret_cast<int32_t*>(lds + plan.pickOff);
     int32_t* ps = reinterpret_cast<int32_t*>(lds + plan.spinOff);
     int32_t* pd = reinterpret_cast<int32_t*>(lds + plan.deltaOff);
-    int32_t* lut = reinterpret_cast<int32_t*>(lds + plan.lutOff);
+    // Static LDS, so the table's address folds into the reads' offset field.
+    __shared__ uint32_t lutb[16];
     int64_t* dpart = reinterpret_cast<int64_t*>(lds + plan.dpartOff);
     int32_t* misc = reinterpret_cast<int32_t*>(lds + plan.miscOff);
 
@@ -181,16 +210,17 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
     const int wave = tid >> 6;
     const int pstride = plan.partStride;
 
-    if (tid < 16) lut[tid] = kGumbelCoarse[tid];
-
     for (;;) {
         if (tid == 0) misc[0] = static_cast<int32_t>(atomicAdd(&sched[0], 1u));
         __syncthreads();
         const int itemIdx = __builtin_amdgcn_readfirstlane(misc[0]);
         if (itemIdx >= nItems) break;
 
+        // Item fields and every per-sweep scalar are wave-uniform; they are
+        // pinned to SGPRs with readfirstlane (the loads go through generic
+        // pointers, which the compiler would otherwise keep in VGPRs).
         const ChainItem& it = items[itemIdx];
-        const int n = it.n;
+        const int n = uni(it.n);
         const uint32_t* __restrict__ offsets = it.offsets;
         const int2* __restrict__ edges = reinterpret_cast<const int2*>(it.edges);
         ChainState* st = it.st;
@@ -198,10 +228,11 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
         const int32_t* betaVal = it.betaVal;
         const uint64_t* phaseEnd = it.phaseEnd;
         const uint32_t* phaseP = it.phaseP;
-        const int nRuns = it.nRuns;
-        const int nPhases = it.nPhases;
-        const uint32_t k0 = static_cast<uint32_t>(it.key);
-        const uint32_t k1 = static_cast<uint32_t>(it.key >> 32);
+        const int nRuns = uni(it.nRuns);
+        const int nPhases = uni(it.nPhases);
+        const uint32_t k0 = uni(static_cast<uint32_t>(it.key));
+        const uint32_t k1 = uni(static_cast<uint32_t>(it.key >> 32));
+
         // ---- load the chain's state into LDS ----
         const int wq = static_cast<int>(wordBytes(n, kBits) / 16);
         const int bq = static_cast<int>(bestBytes(n) / 16);
@@ -213,36 +244,45 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
             uint4* lb = reinterpret_cast<uint4*>(bl);
             for (int i = tid; i < bq; i += NT) lb[i] = gb[i];
         }
-        int64_t cur = st->cur;
-        int64_t best = st->best;
-        uint64_t step = st->step;
-        int improved = st->improved;
-        const uint64_t remaining = it.stepEnd > step ? it.stepEnd - step : 0;
+        int64_t cur = uni(st->cur);
+        int64_t best = uni(st->best);
+        uint64_t step = uni(st->step);
+        int improved = uni(st->improved);
+        const uint64_t itemEnd = uni(it.stepEnd);
+        const uint64_t remaining = itemEnd > step ? itemEnd - step : 0;
         const uint64_t stepEnd = step + (remaining < sliceSweeps ? remaining : sliceSweeps);
         for (int g = tid; g < plan.slotWords; g += NT) part[g] = 0ull;
-        __syncthreads();
 
         // Phase and beta-run cursors for the slice's first sweep.
         int ph = 0;
-        while (ph < nPhases - 1 && phaseEnd[ph] <= step) ++ph;
+        while (ph < nPhases - 1 && uni(phaseEnd[ph]) <= step) ++ph;
+        uint64_t phaseStop = uni(phaseEnd[ph]);
+        int P = static_cast<int>(uni(phaseP[ph]));
         int run = 0;
         {
             int lo = 0, hi = nRuns - 1;
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
-                if (static_cast<uint64_t>(betaEnd[mid]) > step) hi = mid; else lo = mid + 1;
+                if (static_cast<uint64_t>(uni(betaEnd[mid])) > step) hi = mid; else lo = mid + 1;
             }
             run = lo;
         }
+        uint64_t runStop = static_cast<uint64_t>(uni(betaEnd[run]));
+        int32_t beta = uni(betaVal[run]);
+        if (tid < 16)
+            lutb[tid] = static_cast<uint32_t>(kGumbelCoarse[tid]) -
+                        static_cast<uint32_t>(F::kBias) * static_cast<uint32_t>(beta);
+        __syncthreads();
 
         const int nFull = n >> 3;
         const int nBlk = (n + 7) >> 3;
 
         while (step < stepEnd) {
-            while (ph < nPhases - 1 && phaseEnd[ph] <= step) ++ph;
-            while (run < nRuns - 1 && static_cast<uint64_t>(betaEnd[run]) <= step) ++run;
-            const int P = static_cast<int>(phaseP[ph]);
-            const int32_t beta = betaVal[run];
+            if (step >= phaseStop && ph < nPhases - 1) {
+                while (ph < nPhases - 1 && uni(phaseEnd[ph]) <= step) ++ph;
+                phaseStop = uni(phaseEnd[ph]);
+                P = static_cast<int>(uni(phaseP[ph]));
+            }
             const uint64_t s1 = step + 1;   // 1-based sweep id, the Philox counter
             const uint32_t sLo = static_cast<uint32_t>(s1);
             const uint32_t sHi = static_cast<uint32_t>(s1 >> 32);
@@ -250,82 +290,86 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
             const bool laneReduce = pow2 && P <= kMaxLaneReduceP && ((8 * NT) % P) == 0;
 
             // ---------------- SELECT ----------------
-            int32_t accS[8], accV[8];
+            auto select = [&](auto laneTag) {
+                constexpr bool kLane = decltype(laneTag)::value;
+                int32_t accS[8], accV[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) { accS[j] = INT32_MIN; accV[j] = -1; }
+                for (int j = 0; j < 8; ++j) { accS[j] = INT32_MIN; accV[j] = -1; }
+                auto scoreBlock = [&](int b, bool tail) {
+                    Block8<kBits> blk;
+                    blk.load(wl, b);
+                    if (improved) bl8[b] = static_cast<uint8_t>(blk.spinByte());
+                    uint32_t c0 = sLo, c1 = sHi, c2 = static_cast<uint32_t>(b), c3 = 0u;
+                    philox10(c0, c1, c2, c3, k0, k1);
+                    const uint32_t o[4] = {c0, c1, c2, c3};
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int v = 8 * b + j;
+                        if (tail && v >= n) continue;
+                        const uint32_t r = ((j & 1) ? (o[j >> 1] >> 16) : o[j >> 1]);
+                        const uint32_t lb = lutb[r & 15u];
+                        const uint32_t fine = (r >> 4) & 0x7Fu;
+                        const int32_t score = scoreOf<kMul24>(beta, blk.field(j), lb, fine);
+                        if constexpr (kLane) {
+                            // slot j keeps the block index; the vertex is 8*b + j
+                            if (score > accS[j]) { accS[j] = score; accV[j] = b; }
+                        } else {
+                            const int g = pow2 ? (v & (P - 1)) : (v % P);
+                            atomicMax(reinterpret_cast<unsigned long long*>(&part[g]),
+                                      static_cast<unsigned long long>(packKey(score, v)));
+                        }
+                    }
+                };
+                for (int b = tid; b < nFull; b += NT) scoreBlock(b, false);
+                if (nFull < nBlk && tid == (nFull % NT)) scoreBlock(nFull, true);
 
-            auto scoreBlock = [&](int b, bool tail) {
-                Block8<kBits> blk;
-                blk.load(wl, b);
-                if (improved) bl8[b] = static_cast<uint8_t>(blk.spinByte());
-                uint32_t c0 = sLo, c1 = sHi, c2 = static_cast<uint32_t>(b), c3 = 0u;
-                philox10(c0, c1, c2, c3, k0, k1);
-                const uint32_t o[4] = {c0, c1, c2, c3};
+                if constexpr (kLane) {
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int v = 8 * b + j;
-                    if (tail && v >= n) continue;
-                    const uint32_t field = (j & 1) ? (o[j >> 1] >> 16) : o[j >> 1];
-                    const uint32_t r = field & 0x7FFu;
-                    const int32_t gum = lut[r & 15u] + static_cast<int32_t>(r >> 4);
-                    const int32_t delta = wordDelta<kBits>(blk.word(j));
-                    const int32_t bd = kMul24 ? __mul24(beta, delta) : beta * delta;
-                    const int32_t score = gum + bd;
-                    if (laneReduce) {
-                        if (score > accS[j]) { accS[j] = score; accV[j] = v; }
+                    for (int j = 0; j < 8; ++j) accV[j] = accV[j] >= 0 ? 8 * accV[j] + j : -1;
+                    if (P >= 8) {
+                        const int Q = P >> 3;   // lanes l and l' share groups iff l = l' (mod Q)
+                        for (int m = Q; m < 64; m <<= 1) {
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) {
+                                const int32_t s2 = __shfl_xor(accS[j], m, 64);
+                                const int32_t v2 = __shfl_xor(accV[j], m, 64);
+                                keepBetter(accS[j], accV[j], s2, v2);
+                            }
+                        }
+                        if (lane < Q) {
+#pragma unroll
+                            for (int j = 0; j < 8; ++j)
+                                part[wave * pstride + 8 * lane + j] = packKey(accS[j], accV[j]);
+                        }
                     } else {
-                        const int g = pow2 ? (v & (P - 1)) : (v % P);
-                        atomicMax(reinterpret_cast<unsigned long long*>(&part[g]),
-                                  static_cast<unsigned long long>(packKey(score, v)));
+                        // P in {1, 2, 4}: slot j feeds group j mod P.
+                        if (P <= 4) {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) keepBetter(accS[j], accV[j], accS[j + 4], accV[j + 4]);
+                        }
+                        if (P <= 2) {
+#pragma unroll
+                            for (int j = 0; j < 2; ++j) keepBetter(accS[j], accV[j], accS[j + 2], accV[j + 2]);
+                        }
+                        if (P == 1) keepBetter(accS[0], accV[0], accS[1], accV[1]);
+#pragma unroll
+                        for (int m = 1; m < 64; m <<= 1) {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                const int32_t s2 = __shfl_xor(accS[j], m, 64);
+                                const int32_t v2 = __shfl_xor(accV[j], m, 64);
+                                keepBetter(accS[j], accV[j], s2, v2);
+                            }
+                        }
+                        if (lane == 0) {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                if (j < P) part[wave * pstride + j] = packKey(accS[j], accV[j]);
+                        }
                     }
                 }
             };
-            for (int b = tid; b < nFull; b += NT) scoreBlock(b, false);
-            if (nFull < nBlk && tid == (nFull % NT)) scoreBlock(nFull, true);
-
-            if (laneReduce) {
-                if (P >= 8) {
-                    const int Q = P >> 3;   // lanes l and l' share groups iff l = l' (mod Q)
-                    for (int m = Q; m < 64; m <<= 1) {
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            const int32_t s2 = __shfl_xor(accS[j], m, 64);
-                            const int32_t v2 = __shfl_xor(accV[j], m, 64);
-                            keepBetter(accS[j], accV[j], s2, v2);
-                        }
-                    }
-                    if (lane < Q) {
-#pragma unroll
-                        for (int j = 0; j < 8; ++j)
-                            part[wave * pstride + 8 * lane + j] = packKey(accS[j], accV[j]);
-                    }
-                } else {
-                    // P in {1, 2, 4}: slot j feeds group j mod P.
-                    if (P <= 4) {
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) keepBetter(accS[j], accV[j], accS[j + 4], accV[j + 4]);
-                    }
-                    if (P <= 2) {
-#pragma unroll
-                        for (int j = 0; j < 2; ++j) keepBetter(accS[j], accV[j], accS[j + 2], accV[j + 2]);
-                    }
-                    if (P == 1) keepBetter(accS[0], accV[0], accS[1], accV[1]);
-#pragma unroll
-                    for (int m = 1; m < 64; m <<= 1) {
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            const int32_t s2 = __shfl_xor(accS[j], m, 64);
-                            const int32_t v2 = __shfl_xor(accV[j], m, 64);
-                            keepBetter(accS[j], accV[j], s2, v2);
-                        }
-                    }
-                    if (lane == 0) {
-#pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            if (j < P) part[wave * pstride + j] = packKey(accS[j], accV[j]);
-                    }
-                }
-            }
+            if (laneReduce) select(std::true_type{}); else select(std::false_type{});
             __syncthreads();   // B1: all partials / slots written
 
             // ---------------- RESOLVE: winner of each group ----------------
@@ -349,7 +393,7 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                     pick = static_cast<int32_t>(~static_cast<uint32_t>(key));
                     const uint32_t w = ldsWord<kBits>(wl, pick);
                     sp = (w & F::kSpin) ? 1 : -1;
-                    dl = wordDelta<kBits>(w);
+                    dl = static_cast<int32_t>(w & F::kField) - F::kBias;
                 }
                 pk[g] = pick;
                 ps[g] = sp;
@@ -357,45 +401,78 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
             }
             __syncthreads();   // B2: winners published
 
+            // Beta of the next sweep, and its biased LUT (SELECT reads are done).
+            int32_t betaNext = beta;
+            if (runStop <= step + 1 && run < nRuns - 1) {
+                while (run < nRuns - 1 && static_cast<uint64_t>(uni(betaEnd[run])) <= step + 1) ++run;
+                runStop = static_cast<uint64_t>(uni(betaEnd[run]));
+                betaNext = uni(betaVal[run]);
+            }
+            if (tid < 16)
+                lutb[tid] = static_cast<uint32_t>(kGumbelCoarse[tid]) -
+                            static_cast<uint32_t>(F::kBias) * static_cast<uint32_t>(betaNext);
+
             // ---------------- FLIP ----------------
             int64_t dsum = 0;
             {
                 int lgL = 0;
-                if (P < NT) lgL = 31 - __clz(NT / P);
+                if (P < NT) {
+                    lgL = 31 - __clz(NT / P);
+                    lgL = lgL > 6 ? 6 : lgL;   // a winner's lanes stay inside one wave
+                }
                 const int L = 1 << lgL;
+                const int perPass = NT >> lgL;
+                const int passes = (P + perPass - 1) / perPass;
                 const int sub = tid & (L - 1);
-                for (int g = tid >> lgL; g < P; g += (NT >> lgL)) {
-                    const int32_t b = pk[g];
-                    if (b < 0) continue;
-                    const int32_t sb = ps[g];
-                    const uint32_t lo = offsets[b];
-                    const uint32_t hi = offsets[b + 1];
-                    for (uint32_t i = lo + sub; i < hi; i += L) {
-                        const int2 e = edges[i];
-                        const int x = e.x;
-                        const int32_t w2 = 2 * e.y;
-                        const int gx = pow2 ? (x & (P - 1)) : (x % P);
-                        // Earlier winner adjacent to b: the coupling term that
-                        // sequential group order would have seen in S[b].
-                        if (gx < g && pk[gx] == x)
-                            dsum -= static_cast<int64_t>(w2) * (sb * ps[gx]);
-                        ldsAddField<kBits>(wl, x, -w2 * sb);
+                for (int pass = 0; pass < passes; ++pass) {
+                    const int g = pass * perPass + (tid >> lgL);
+                    int32_t b = -1, sb = 0;
+                    if (g < P) { b = pk[g]; sb = ps[g]; }
+                    int32_t own = 0;   // sum over adjacent winners x of 2 w s_b s_x
+                    if (b >= 0) {
+                        const uint32_t lo = offsets[b];
+                        const uint32_t hi = offsets[b + 1];
+                        for (uint32_t i = lo + sub; i < hi; i += L) {
+                            const int2 e = edges[i];
+                            const int x = e.x;
+                            const int32_t w2 = 2 * e.y;
+                            const int gx = pow2 ? (x & (P - 1)) : (x % P);
+                            if (pk[gx] == x) {
+                                // x flips too: its coupling enters b's own
+                                // update, and, for an earlier group, the cut
+                                // delta as sequential group order sees it.
+                                const int32_t t = w2 * sb * ps[gx];
+                                own += t;
+                                if (gx < g) dsum -= t;
+                            } else {
+                                // delta(x) += s_x * (-2 w s_b); x's spin is stable.
+                                const bool up = (ldsWord<kBits>(wl, x) & F::kSpin) != 0;
+                                const int32_t d = up ? -w2 * sb : w2 * sb;
+                                ldsAddWord<kBits>(wl, x, static_cast<uint32_t>(d));
+                            }
+                        }
                     }
-                    if (sub == 0) {
+#pragma unroll
+                    for (int m = 1; m < 64; m <<= 1)
+                        if (m < L) own += __shfl_xor(own, m, 64);
+                    if (b >= 0 && sub == 0) {
                         dsum += pd[g];
-                        ldsFlipSpin<kBits>(wl, b);
+                        // delta(b) -> -delta(b) + own, spin bit toggled: one add.
+                        const uint32_t toggle = sb > 0 ? 0u - F::kSpin : F::kSpin;
+                        ldsAddWord<kBits>(wl, b, static_cast<uint32_t>(own) - 2u * static_cast<uint32_t>(pd[g]) + toggle);
                     }
                 }
             }
             dsum = waveSum64(dsum);
             if (lane == 0) dpart[wave] = dsum;
-            __syncthreads();   // B3: spins, fields and partial sums complete
+            __syncthreads();   // B3: spins, fields, LUT and partial sums complete
 
             int64_t total = 0;
 #pragma unroll
             for (int w = 0; w < NW; ++w) total += dpart[w];
-            cur += total;
+            cur += uni(total);
             if (cur > best) { best = cur; improved = 1; } else { improved = 0; }
+            beta = betaNext;
             ++step;
         }
 
@@ -441,8 +518,9 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
 }
 
 // Packs initial spins into state words: S[v] = sum_w w * spin[to] over v's
-// sorted row (src/anneal_kernels.cu:251-256), word = spin bit | S + bias;
-// best-spin bitmask = initial spins.  One thread per 32 vertices.
+// sorted row (src/anneal_kernels.cu:251-256), word = spin bit |
+// spin*S + bias; best-spin bitmask = initial spins.  One thread per 32
+// vertices.
 template <int kBits>
 __global__ __launch_bounds__(256) void initWords(const InitItem* __restrict__ items, int nItems) {
     using F = WordFmt<kBits>;
@@ -467,7 +545,8 @@ __global__ __launch_bounds__(256) void initWords(const InitItem* __restrict__ it
                     S += e.y * static_cast<int32_t>(it.spins[e.x]);
                 }
                 const bool up = it.spins[v] > 0;
-                word = (up ? F::kSpin : 0u) | static_cast<uint32_t>(S + F::kBias);
+                const int32_t delta = up ? S : -S;
+                word = (up ? F::kSpin : 0u) | static_cast<uint32_t>(delta + F::kBias);
                 bits |= (up ? 1u : 0u) << k;
             }
             if constexpr (kBits == 32) {
@@ -496,17 +575,17 @@ __global__ __launch_bounds__(256) void streamCopy(const uint4* __restrict__ src,
 
 namespace launch {
 
+using dev::ChainItem;
+using dev::InitItem;
+using dev::LdsPlan;
+using dev::kThreads;
+
 hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s) {
     const size_t n16 = bytes / 16;
     hipLaunchKernelGGL(dev::streamCopy, dim3(4096), dim3(256), 0, s, static_cast<const uint4*>(src),
                        static_cast<uint4*>(dst), n16);
     return hipGetLastError();
 }
-
-using dev::ChainItem;
-using dev::InitItem;
-using dev::LdsPlan;
-using dev::kThreads;
 
 template <int kBits, bool kMul24>
 static hipError_t launchAnneal(const ChainItem* items, int nItems, uint32_t* sched,
@@ -529,8 +608,7 @@ hipError_t anneal(int bits, bool mul24, const ChainItem* items, int nItems, uint
                               : launchAnneal<8, false>(items, nItems, sched, plan, grid, sliceSweeps, s);
         case 16: return mul24 ? launchAnneal<16, true>(items, nItems, sched, plan, grid, sliceSweeps, s)
                               : launchAnneal<16, false>(items, nItems, sched, plan, grid, sliceSweeps, s);
-        case 32: return mul24 ? launchAnneal<32, true>(items, nItems, sched, plan, grid, sliceSweeps, s)
-                              : launchAnneal<32, false>(items, nItems, sched, plan, grid, sliceSweeps, s);
+        case 32: return launchAnneal<32, false>(items, nItems, sched, plan, grid, sliceSweeps, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -552,7 +630,7 @@ int maxActiveBlocks(int bits, bool mul24, size_t lds) {
     switch (bits) {
         case 8:  return mul24 ? occupancyOf<8, true>(lds) : occupancyOf<8, false>(lds);
         case 16: return mul24 ? occupancyOf<16, true>(lds) : occupancyOf<16, false>(lds);
-        case 32: return mul24 ? occupancyOf<32, true>(lds) : occupancyOf<32, false>(lds);
+        case 32: return occupancyOf<32, false>(lds);
         default: return 0;
     }
 }

@@ -14,6 +14,8 @@ namespace dev {
 // Threads per workgroup of the slice kernel (4 wave64).
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
+// Static LDS of the slice kernel (the 16-entry biased Gumbel LUT).
+constexpr uint32_t kStaticLds = 64;
 
 // Byte offsets of the LDS regions of one workgroup (host-computed from the
 // largest graph and the largest P of the launch).
@@ -24,7 +26,7 @@ struct LdsPlan {
     uint32_t pickOff;     // int32 winner id per group
     uint32_t spinOff;     // int32 winner pre-flip spin per group
     uint32_t deltaOff;    // int32 winner pre-flip delta per group
-    uint32_t lutOff;      // int32[16] Gumbel LUT
+    uint32_t lutOff;      // unused (the LUT is static LDS)
     uint32_t dpartOff;    // int64 per-wave cut-delta partials
     uint32_t miscOff;     // scheduler broadcast
     uint32_t total;       // dynamic LDS bytes
@@ -50,13 +52,13 @@ inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP) {
     const int laneP = maxP < kMaxLaneReduceP ? maxP : kMaxLaneReduceP;
     const int slots = (kWaves * laneP > maxP) ? kWaves * laneP : maxP;
     uint32_t off = 0;
+    p.lutOff = 0;     // the Gumbel LUT is a static 64 B LDS array of the kernel
     p.wordsOff = off; off += alignUp(static_cast<uint32_t>(wordBytes(maxN, bits)), 16);
     p.bestOff = off;  off += alignUp(static_cast<uint32_t>(bestBytes(maxN)), 16);
     p.partOff = off;  off += alignUp(static_cast<uint32_t>(slots) * 8u, 16);
     p.pickOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
     p.spinOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
     p.deltaOff = off; off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
-    p.lutOff = off;   off += 64;
     p.dpartOff = off; off += alignUp(kWaves * 8u, 16);
     p.miscOff = off;  off += 16;
     p.total = off;

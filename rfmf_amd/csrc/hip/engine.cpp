@@ -206,9 +206,10 @@ std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t str
     job->bits = wordBitsFor(g->B);
     if (job->bits == 0)
         throw Error(Status::Unsupported, "max weighted degree >= 2^30 is not supported");
-    job->mul24 = betaMax < (int64_t{1} << 23) && g->B < (int64_t{1} << 23);
+    // 24-bit multiplies need |beta| < 2^23 and a field below 2^23 (8/16-bit words).
+    job->mul24 = betaMax < (int64_t{1} << 23) && job->bits <= 16;
     const dev::LdsPlan plan = dev::makeLdsPlan(n, job->bits, maxP);
-    if (static_cast<size_t>(plan.total) > props_.sharedMemPerBlock)
+    if (static_cast<size_t>(plan.total) + dev::kStaticLds > props_.sharedMemPerBlock)
         throw Error(Status::Unsupported, "chain state of n=" + std::to_string(n) + " (" +
                                              std::to_string(plan.total) +
                                              " B) exceeds the LDS per workgroup");
@@ -327,7 +328,7 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
         g.bits = (gi / 2 == 0) ? 8 : (gi / 2 == 1) ? 16 : 32;
         g.mul24 = (gi % 2) == 1;
         g.plan = dev::makeLdsPlan(maxN[gi], g.bits, maxP[gi]);
-        if (static_cast<size_t>(g.plan.total) > props_.sharedMemPerBlock)
+        if (static_cast<size_t>(g.plan.total) + dev::kStaticLds > props_.sharedMemPerBlock)
             throw Error(Status::Unsupported, "combined launch needs " + std::to_string(g.plan.total) +
                                                  " B of LDS per workgroup");
         const int occ = std::max(1, occupancy(g.bits, g.mul24, g.plan.total));

@@ -66,7 +66,17 @@ def main() -> int:
     ap.add_argument("--cpu-sweeps", type=int, default=4000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (counter calibration) and exit")
+    ap.add_argument("--config", default="torus", choices=["torus", "er800", "sparse", "batch", "serve"],
+                    help="torus: the headline plain run (default); the others are the other workloads")
+    ap.add_argument("--threads", type=int, default=0, help="threads per chain (0: autotune)")
+    ap.add_argument("--qps", type=float, default=20.0, help="serve: Poisson arrival rate")
+    ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
+    ap.add_argument("--graphs", type=int, default=256, help="batch: number of graphs")
     args = ap.parse_args()
+
+    if args.config != "torus" and not args.copy_probe:
+        return {"er800": bench_er800, "sparse": bench_sparse, "batch": bench_batch,
+                "serve": bench_serve}[args.config](args)
 
     if args.copy_probe:
         import rfmf_amd as R
@@ -162,6 +172,189 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _job(R, csr, seeds, iters, beta_range=(0.1, 3.0)):
+    init = np.stack([R.generate_random_spins(s, csr.n) for s in seeds])
+    solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+    beta = R.build_linear_beta_table(beta_range[0], beta_range[1], sum(iters), len(seeds))
+    return R.Job(csr, init, solver, beta, LADDER, iters)
+
+
+def _emit(obj):
+    print(json.dumps(obj), flush=True)
+
+
+def bench_er800(args) -> int:
+    """ER n=800 p=0.06 unit weights, 64 chains, 1M sweeps (linear beta 0.1->3.0,
+    P ladder 128..1): GPU run and the CPU oracle on every chain, compared bit
+    for bit per seed."""
+    import rfmf_amd as R
+    from rfmf_amd import workloads as W
+    from oracle import oracle as O
+    sweeps = args.steps if args.steps != 40000 else 1_000_000
+    csr = W.erdos_renyi(800, 0.06, seed=0)
+    C = 64
+    iters = [sweeps // 8] * 8
+    job = _job(R, csr, range(1, C + 1), iters)
+    log(f"[er800] n={csr.n} m={csr.m2 // 2} B={R.max_weighted_degree(csr)} chains={C} sweeps={sum(iters)}")
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ms = ctx.batch_run([(gid, job)], autotune=args.threads == 0, threads=args.threads)
+        geo = ctx.last_launch()
+    gpu_rate = C * sum(iters) / (ms / 1e3)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(16, cores))
+    end, best, cv, bv, sec = O.anneal_many(csr.offsets, csr.edge_to, csr.edge_weight, job.init, job.seeds,
+                                           job.beta.run_end, job.beta.run_val, job.beta.chain_run_off,
+                                           LADDER, iters, threads=threads)
+    same = (np.array_equal(end, job.end_spins) and np.array_equal(best, job.best_spins) and
+            np.array_equal(cv, job.final_vals) and np.array_equal(bv, job.best_vals))
+    recomputed = all(R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c] for c in range(C))
+    log(f"[er800] gpu {ms:.1f} ms ({gpu_rate:.3e} chain-steps/s, {geo}); cpu {sec:.1f} s on {threads} threads; "
+        f"bit-exact per seed: {same}")
+    _emit({"metric": "chain-steps/sec", "config": "er800_p0.06_64chains", "sweeps": sum(iters),
+           "gpu_value": gpu_rate, "gpu_kernel_ms": ms, "geometry": geo,
+           "cpu_value": C * sum(iters) / sec, "cpu_threads": threads, "cpu_seconds": sec,
+           "bit_exact_per_seed": bool(same), "best_cut_recomputed": bool(recomputed),
+           "best_cut_median": float(np.median(job.best_vals)), "best_cut_max": int(job.best_vals.max())})
+    return 0 if same and recomputed else 1
+
+
+def bench_sparse(args) -> int:
+    """Sparse random graphs, 4096 chains each in one persistent launch with the
+    autotuned threads-per-chain: n=10000 m=10000 unit weights, and n=7000
+    m=17150 +/-1."""
+    import rfmf_amd as R
+    from rfmf_amd import workloads as W
+    sweeps = args.steps if args.steps != 40000 else 100_000
+    iters = [sweeps // 8] * 8
+    C = 4096
+    rc = 0
+    for name, csr in (("sparse_n10000_m10000_unit", W.sparse_random(10000, 10000, False, seed=0)),
+                      ("sparse_n7000_m17150_pm1", W.sparse_random(7000, 17150, True, seed=0))):
+        job = _job(R, csr, range(1, C + 1), iters)
+        with R.Context(0) as ctx:
+            gid = ctx.upload_graph(csr)
+            ms = ctx.batch_run([(gid, job)], autotune=args.threads == 0, threads=args.threads)
+            geo = ctx.last_launch()
+        ok = all(R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c] for c in range(0, C, 97))
+        rc |= 0 if ok else 1
+        rate = C * sum(iters) / (ms / 1e3)
+        log(f"[sparse] {name}: n={csr.n} {ms:.1f} ms {rate:.3e} chain-steps/s {geo}")
+        _emit({"metric": "chain-steps/sec", "config": name, "n": csr.n, "m": csr.m2 // 2, "chains": C,
+               "sweeps": sum(iters), "value": rate, "kernel_ms": ms, "geometry": geo,
+               "best_cut_median": float(np.median(job.best_vals)), "best_cut_max": int(job.best_vals.max()),
+               "cuts_recomputed": bool(ok)})
+    return rc
+
+
+def bench_batch(args) -> int:
+    """Batched mixed instances: `graphs` independent graphs (n log-uniform in
+    [800, 20000], family uniform over ER mean degree 6 / torus +/-1 / random
+    degree 2-6 +/-1), 64 chains each, in one hipGraph-captured persistent
+    launch, against the plain mode: one launch per graph, graphs one after
+    another.  Every chain must be identical in both modes."""
+    import rfmf_amd as R
+    from rfmf_amd import workloads as W
+    sweeps = args.steps if args.steps != 40000 else 20_000
+    iters = [sweeps // 8] * 8
+    t0 = time.perf_counter()
+    graphs = W.batch_instances(args.graphs, seed=0)
+    log(f"[batch] generated {len(graphs)} graphs in {time.perf_counter() - t0:.1f} s; "
+        f"sum n = {sum(g.n for _, g in graphs)}")
+    jobs_b = [_job(R, g, range(64 * i + 1, 64 * i + 65), iters) for i, (_, g) in enumerate(graphs)]
+    jobs_p = [_job(R, g, range(64 * i + 1, 64 * i + 65), iters) for i, (_, g) in enumerate(graphs)]
+    work = 64 * len(graphs) * sum(iters)
+    with R.Context(0) as ctx:
+        gids = [ctx.upload_graph(g) for _, g in graphs]
+        ms_b = ctx.batch_run(list(zip(gids, jobs_b)), slice_sweeps=sum(iters), use_hip_graph=True,
+                             autotune=args.threads == 0, threads=args.threads)
+        geo = ctx.last_launch()
+        ms_p = 0.0
+        for gid, j in zip(gids, jobs_p):
+            ms_p += ctx.batch_run([(gid, j)])
+    same = all(np.array_equal(a.end_spins, b.end_spins) and np.array_equal(a.best_spins, b.best_spins) and
+               np.array_equal(a.best_vals, b.best_vals) for a, b in zip(jobs_b, jobs_p))
+    rb, rp = work / (ms_b / 1e3), work / (ms_p / 1e3)
+    med_b = float(np.median(np.concatenate([j.best_vals for j in jobs_b])))
+    med_p = float(np.median(np.concatenate([j.best_vals for j in jobs_p])))
+    log(f"[batch] batched {ms_b:.1f} ms {rb:.3e} chain-steps/s ({geo}); plain {ms_p:.1f} ms {rp:.3e}; "
+        f"x{rb / rp:.2f}; identical: {same}")
+    _emit({"metric": "chain-steps/sec", "config": f"batch_{len(graphs)}graphs_64chains", "sweeps": sum(iters),
+           "batched_value": rb, "plain_value": rp, "speedup": rb / rp, "batched_kernel_ms": ms_b,
+           "plain_kernel_ms": ms_p, "geometry": geo, "identical_results": bool(same),
+           "best_cut_median_batched": med_b, "best_cut_median_plain": med_p})
+    return 0 if same else 1
+
+
+def bench_serve(args) -> int:
+    """Serving: a Poisson stream of solve requests at `qps`, each a seeded
+    graph from the batched-mix generator (n 800-20000) with 32-256 chains and
+    a log-uniform sweep count, admitted by HBM footprint into a continuously
+    batched server, against plain per-request launches served one at a time
+    in arrival order.  Reports p50/p99 latency and aggregate chain-steps/s;
+    every request's result must be identical in both modes."""
+    import rfmf_amd as R
+    from rfmf_amd import workloads as W
+    rng = np.random.default_rng(2026)
+    nreq = args.requests
+    graphs = W.batch_instances(nreq, seed=7)
+    lo, hi = 10_000, 100_000
+    reqs = []
+    for i, (_, g) in enumerate(graphs):
+        chains = int(rng.integers(32, 257))
+        sweeps = int(np.exp(rng.uniform(np.log(lo), np.log(hi)))) // 8 * 8
+        reqs.append((g, chains, sweeps))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, nreq))
+    work = float(sum(c * s for _, c, s in reqs))
+
+    def mk(i, g, chains, sweeps):
+        return _job(R, g, range(1000 * i + 1, 1000 * i + 1 + chains), [sweeps // 8] * 8)
+
+    served = [mk(i, *r) for i, r in enumerate(reqs)]
+    plain = [mk(i, *r) for i, r in enumerate(reqs)]
+    log(f"[serve] {nreq} requests, {args.qps} qps, sweeps log-uniform [{lo}, {hi}], total {work:.3e} chain-steps")
+    with R.Context(0) as ctx:
+        # plain: each request alone, wall time of its whole call (upload, run, download)
+        svc = []
+        for j in plain:
+            t = time.perf_counter()
+            gid = ctx.upload_graph(j.csr)
+            ctx.batch_run([(gid, j)])
+            ctx.release_graph(gid)
+            svc.append(time.perf_counter() - t)
+        with R.Server(ctx, hbm_budget_bytes=64 << 30, slice_sweeps=500) as srv:
+            t0 = time.perf_counter()
+            ids = []
+            for i, j in enumerate(served):
+                delay = arrivals[i] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(srv.submit(j))
+            for rid in ids:
+                srv.wait(rid, 1200.0)
+            polls = [srv.poll(rid) for rid in ids]
+            st = srv.stats()
+    lat_s = np.array([td - ts for _, ts, _, td in polls])
+    span_s = max(td for *_, td in polls) - min(ts for _, ts, _, _ in polls)
+    finish, start_free = [], 0.0
+    for a, s in zip(arrivals, svc):
+        start = max(a, start_free)
+        start_free = start + s
+        finish.append(start_free)
+    lat_p = np.array(finish) - arrivals
+    span_p = finish[-1] - arrivals[0]
+    same = all(np.array_equal(a.end_spins, b.end_spins) and np.array_equal(a.best_spins, b.best_spins)
+               for a, b in zip(served, plain))
+    out = {"metric": "chain-steps/sec and solve latency", "config": f"serve_{nreq}req_{args.qps}qps",
+           "served_value": work / span_s, "plain_value": work / span_p, "speedup": span_p / span_s,
+           "served_p50_s": float(np.percentile(lat_s, 50)), "served_p99_s": float(np.percentile(lat_s, 99)),
+           "plain_p50_s": float(np.percentile(lat_p, 50)), "plain_p99_s": float(np.percentile(lat_p, 99)),
+           "slices": st["slices"], "busy_kernel_ms": st["busy_kernel_ms"], "identical_results": bool(same)}
+    log(f"[serve] {out}")
+    _emit(out)
+    return 0 if same else 1
 
 
 def run_two_slices(ctx, gid, job, warm: int, timed: int):

@@ -200,12 +200,22 @@ typedef struct rfmf_job {
 /* Launch knobs; every one is output-invariant.  Zero fields mean "auto"
  * (the autotuner's cached choice, else the built-in default). */
 typedef struct rfmf_tuning {
-    int64_t slice_sweeps;    /* sweeps per launch (0: whole run in one launch) */
-    int32_t grid_cap;        /* max resident workgroups (0: occupancy x CUs) */
-    int32_t use_hip_graph;   /* 1: replay the slice launches from a captured hipGraph */
-    int32_t xcd_affinity;    /* 1: keep each graph's chains on one XCD where possible */
-    int32_t reserved[3];
+    int64_t slice_sweeps;       /* sweeps per launch (0: whole run in one launch) */
+    int32_t grid_cap;           /* max resident workgroups (0: occupancy x CUs) */
+    int32_t use_hip_graph;      /* 1: replay the slice launches from a captured hipGraph */
+    int32_t threads_per_chain;  /* 128/256/512/1024 (2..16 wave64 per chain); 0: tuned or heuristic */
+    int32_t autotune;           /* 1: time every thread count on one short slice of the live run
+                                   first (exact: geometry never changes a result), cache the
+                                   fastest per launch shape, run the rest with it */
+    int32_t reserved[2];
 } rfmf_tuning;
+
+/* Geometry of the context's last batch launch: threads per chain, grid,
+ * state word bits, resident workgroups per CU, and the autotuner's trial
+ * ms per sweep for 128/256/512/1024 threads (0 where not tried).  Replaces the
+ * reference's hard-coded V100 launch constants (src/tuning.hpp:8-18). */
+rfmf_status rfmf_ctx_last_launch(rfmf_ctx* ctx, int32_t* threads, int32_t* grid, int32_t* word_bits,
+                                 int32_t* occupancy, float* trial_ms_per_sweep);
 
 /* Runs every job's chains to completion in persistent launches that pack all
  * jobs together; returns the kernel-only time of all launches. */

@@ -293,6 +293,11 @@ class Job:
                          _ptr(self.best_vals, C.c_int64), _ptr(self.final_vals, C.c_int64))
 
 
+def _tuning(slice_sweeps=0, grid_cap=0, use_hip_graph=False, threads=0, autotune=False) -> _capi.Tuning:
+    return _capi.Tuning(int(slice_sweeps), int(grid_cap), int(bool(use_hip_graph)), int(threads),
+                        int(bool(autotune)), (C.c_int32 * 2)())
+
+
 class Context:
     """Resident device context: graphs stay in HBM between batch runs."""
 
@@ -347,13 +352,21 @@ class Context:
         self._graphs.pop(gid, None)
 
     def batch_run(self, jobs: list[tuple[int, Job]], slice_sweeps: int = 0, grid_cap: int = 0,
-                  use_hip_graph: bool = False) -> float:
+                  use_hip_graph: bool = False, threads: int = 0, autotune: bool = False) -> float:
         """Runs (graph_id, Job) pairs to completion in packed persistent launches."""
         arr = (_capi.Job * len(jobs))(*[j.struct(gid) for gid, j in jobs])
-        tun = _capi.Tuning(int(slice_sweeps), int(grid_cap), int(bool(use_hip_graph)), 0, (C.c_int32 * 3)())
+        tun = _tuning(slice_sweeps, grid_cap, use_hip_graph, threads, autotune)
         ms = C.c_float()
         check(lib().rfmf_batch_run(self._h, arr, len(jobs), C.byref(tun), C.byref(ms)))
         return ms.value
+
+    def last_launch(self) -> dict:
+        """Geometry of the last batch launch and the autotuner's trial times."""
+        t, g, b, o = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        trial = (C.c_float * 4)()
+        check(lib().rfmf_ctx_last_launch(self._h, C.byref(t), C.byref(g), C.byref(b), C.byref(o), trial))
+        return {"threads_per_chain": t.value, "grid": g.value, "word_bits": b.value, "wg_per_cu": o.value,
+                "trial_ms_per_sweep": {n: trial[i] for i, n in enumerate((128, 256, 512, 1024)) if trial[i] > 0}}
 
     def batch_run_windows(self, jobs: list[tuple[int, Job]], bounds: list[int]) -> list[float]:
         """Runs jobs with one timed launch per window [bounds[i-1], bounds[i])
@@ -375,8 +388,9 @@ class Run:
         self._h = C.c_void_p()
         check(lib().rfmf_run_create(ctx.handle, self._arr, len(jobs), C.byref(self._h)))
 
-    def advance(self, sweeps: int, use_hip_graph: bool = False, grid_cap: int = 0) -> tuple[float, bool]:
-        tun = _capi.Tuning(0, int(grid_cap), int(bool(use_hip_graph)), 0, (C.c_int32 * 3)())
+    def advance(self, sweeps: int, use_hip_graph: bool = False, grid_cap: int = 0,
+                threads: int = 0) -> tuple[float, bool]:
+        tun = _tuning(0, grid_cap, use_hip_graph, threads, False)
         ms = C.c_float()
         fin = C.c_int32()
         check(lib().rfmf_run_advance(self._h, int(sweeps), C.byref(tun), C.byref(ms), C.byref(fin)))

@@ -587,52 +587,48 @@ hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s) {
     return hipGetLastError();
 }
 
+// The compiled variants: word width x 24-bit multiply x threads per chain.
 template <int kBits, bool kMul24>
-static hipError_t launchAnneal(const ChainItem* items, int nItems, uint32_t* sched,
-                               const LdsPlan& plan, int grid, uint64_t sliceSweeps,
-                               hipStream_t s) {
-    auto kern = dev::annealSlice<kBits, kMul24, kThreads>;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+static const void* kernelFor(int threads) {
+    switch (threads) {
+        case 128: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMul24, 128>);
+        case 256: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMul24, 256>);
+        case 512: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMul24, 512>);
+        case 1024: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMul24, 1024>);
+        default: return nullptr;
+    }
+}
+
+static const void* kernelFor(int bits, bool mul24, int threads) {
+    switch (bits) {
+        case 8:  return mul24 ? kernelFor<8, true>(threads) : kernelFor<8, false>(threads);
+        case 16: return mul24 ? kernelFor<16, true>(threads) : kernelFor<16, false>(threads);
+        case 32: return kernelFor<32, false>(threads);
+        default: return nullptr;
+    }
+}
+
+hipError_t anneal(int bits, bool mul24, int threads, const ChainItem* items, int nItems, uint32_t* sched,
+                  const LdsPlan& plan, int grid, uint64_t sliceSweeps, hipStream_t s) {
+    const void* kern = kernelFor(bits, mul24, threads);
+    if (!kern) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        static_cast<int>(plan.total));
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), plan.total, s, items, nItems, sched, plan,
-                       sliceSweeps);
-    return hipGetLastError();
+    LdsPlan p = plan;
+    void* args[] = {(void*)&items, (void*)&nItems, (void*)&sched, (void*)&p, (void*)&sliceSweeps};
+    return hipLaunchKernel(kern, dim3(grid), dim3(threads), args, plan.total, s);
 }
 
-hipError_t anneal(int bits, bool mul24, const ChainItem* items, int nItems, uint32_t* sched,
-                  const LdsPlan& plan, int grid, uint64_t sliceSweeps, hipStream_t s) {
-    switch (bits) {
-        case 8:  return mul24 ? launchAnneal<8, true>(items, nItems, sched, plan, grid, sliceSweeps, s)
-                              : launchAnneal<8, false>(items, nItems, sched, plan, grid, sliceSweeps, s);
-        case 16: return mul24 ? launchAnneal<16, true>(items, nItems, sched, plan, grid, sliceSweeps, s)
-                              : launchAnneal<16, false>(items, nItems, sched, plan, grid, sliceSweeps, s);
-        case 32: return launchAnneal<32, false>(items, nItems, sched, plan, grid, sliceSweeps, s);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-template <int kBits, bool kMul24>
-static int occupancyOf(size_t lds) {
+int maxActiveBlocks(int bits, bool mul24, int threads, size_t lds) {
+    const void* kern = kernelFor(bits, mul24, threads);
+    if (!kern) return 0;
+    if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) !=
+        hipSuccess)
+        return 0;
     int blocks = 0;
-    auto kern = dev::annealSlice<kBits, kMul24, kThreads>;
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            static_cast<int>(lds)) != hipSuccess)
-        return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, kThreads, lds) != hipSuccess)
-        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, threads, lds) != hipSuccess) return 0;
     return blocks;
-}
-
-int maxActiveBlocks(int bits, bool mul24, size_t lds) {
-    switch (bits) {
-        case 8:  return mul24 ? occupancyOf<8, true>(lds) : occupancyOf<8, false>(lds);
-        case 16: return mul24 ? occupancyOf<16, true>(lds) : occupancyOf<16, false>(lds);
-        case 32: return occupancyOf<32, false>(lds);
-        default: return 0;
-    }
 }
 
 hipError_t initWords(int bits, const InitItem* items, int nItems, int maxN, hipStream_t s) {

@@ -11,9 +11,11 @@
 namespace rfmf {
 namespace dev {
 
-// Threads per workgroup of the slice kernel (4 wave64).
+// Threads per workgroup (= per resident chain) the slice kernel is built
+// for: 2, 4, 8 or 16 wave64.  256 is the default tile.
+constexpr int kThreadChoices[4] = {128, 256, 512, 1024};
 constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
+inline bool validThreads(int nt) { return nt == 128 || nt == 256 || nt == 512 || nt == 1024; }
 // Static LDS of the slice kernel (the 16-entry biased Gumbel LUT).
 constexpr uint32_t kStaticLds = 64;
 
@@ -47,10 +49,11 @@ struct InitItem {
 
 inline uint32_t alignUp(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
 
-inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP) {
+inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads) {
     LdsPlan p{};
+    const int waves = threads / 64;
     const int laneP = maxP < kMaxLaneReduceP ? maxP : kMaxLaneReduceP;
-    const int slots = (kWaves * laneP > maxP) ? kWaves * laneP : maxP;
+    const int slots = (waves * laneP > maxP) ? waves * laneP : maxP;
     uint32_t off = 0;
     p.lutOff = 0;     // the Gumbel LUT is a static 64 B LDS array of the kernel
     p.wordsOff = off; off += alignUp(static_cast<uint32_t>(wordBytes(maxN, bits)), 16);
@@ -59,7 +62,7 @@ inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP) {
     p.pickOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
     p.spinOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
     p.deltaOff = off; off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
-    p.dpartOff = off; off += alignUp(kWaves * 8u, 16);
+    p.dpartOff = off; off += alignUp(static_cast<uint32_t>(waves) * 8u, 16);
     p.miscOff = off;  off += 16;
     p.total = off;
     p.partStride = laneP;
@@ -72,14 +75,14 @@ inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP) {
 namespace launch {
 
 // Persistent slice kernel over `nItems` chain items, each advanced by at most
-// `sliceSweeps` sweeps; `sched` is a 2-word device counter pair, zero before
-// the launch and left zero after it.
-hipError_t anneal(int bits, bool mul24, const dev::ChainItem* items, int nItems,
+// `sliceSweeps` sweeps, `threads` threads per workgroup; `sched` is a 2-word
+// device counter pair, zero before the launch and left zero after it.
+hipError_t anneal(int bits, bool mul24, int threads, const dev::ChainItem* items, int nItems,
                   uint32_t* sched, const dev::LdsPlan& plan, int grid, uint64_t sliceSweeps,
                   hipStream_t s);
 
 // Resident workgroups per CU for a variant at a given dynamic LDS size.
-int maxActiveBlocks(int bits, bool mul24, size_t lds);
+int maxActiveBlocks(int bits, bool mul24, int threads, size_t lds);
 
 // Device-to-device copy of `bytes` (a multiple of 16) with a plain kernel.
 hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s);

@@ -255,7 +255,10 @@ void annealOneShot(const rfmf_csr* g, int64_t B, const int8_t* init, int64_t C,
     spec.bestVals = bestVals;
     spec.finalVals = finalVals;
     auto job = ctx.admit(spec, ctx.stream());
+    // The drop-in path picks its geometry with the autotuner (the trial
+    // slices are part of the run, so the result is unchanged).
     rfmf::LaunchKnobs knobs;
+    knobs.autotune = true;
     const float t = ctx.runToCompletion({job.get()}, knobs);
     if (ms) *ms = t;
     ctx.collect(*job, spec, ctx.stream());
@@ -369,6 +372,21 @@ rfmf_status rfmf_ctx_device_info(rfmf_ctx* ctx, int32_t* cu_count, int64_t* lds_
     });
 }
 
+rfmf_status rfmf_ctx_last_launch(rfmf_ctx* ctx, int32_t* threads, int32_t* grid, int32_t* word_bits,
+                                 int32_t* occupancy, float* trial_ms_per_sweep) {
+    return guarded([&] {
+        if (!ctx) throw Error(Status::InvalidArg, "null context");
+        std::lock_guard<std::mutex> lk(ctx->ctx->mutex());
+        const rfmf::LaunchReport& r = ctx->ctx->lastLaunch();
+        if (threads) *threads = r.threads;
+        if (grid) *grid = r.grid;
+        if (word_bits) *word_bits = r.bits;
+        if (occupancy) *occupancy = r.occupancy;
+        if (trial_ms_per_sweep)
+            for (int i = 0; i < 4; ++i) trial_ms_per_sweep[i] = r.trialMsPerSweep[i];
+    });
+}
+
 rfmf_status rfmf_ctx_copy_probe(rfmf_ctx* ctx, int64_t bytes, int32_t iters, float* avg_ms) {
     return guarded([&] {
         if (!ctx || bytes < 16 || iters <= 0 || !avg_ms) throw Error(Status::InvalidArg, "invalid probe");
@@ -427,9 +445,15 @@ rfmf::JobSpec specFromJob(const rfmf_job& j, const rfmf::DevGraph* g) {
 rfmf::LaunchKnobs knobsFrom(const rfmf_tuning* t) {
     rfmf::LaunchKnobs k;
     if (t) {
+        if (t->slice_sweeps < 0 || t->grid_cap < 0)
+            throw Error(Status::InvalidArg, "negative slice length or grid cap");
+        if (t->threads_per_chain != 0 && !rfmf::dev::validThreads(t->threads_per_chain))
+            throw Error(Status::InvalidArg, "threads_per_chain must be 0, 128, 256, 512 or 1024");
         k.sliceSweeps = t->slice_sweeps;
         k.gridCap = t->grid_cap;
         k.hipGraph = t->use_hip_graph != 0;
+        k.threads = t->threads_per_chain;
+        k.autotune = t->autotune != 0;
     }
     return k;
 }
@@ -506,7 +530,7 @@ rfmf_status rfmf_run_advance(rfmf_run* run, int64_t sweeps, const rfmf_tuning* t
         if (active > 0) {
             if (k.hipGraph) {
                 if (!run->graph || run->graphSlice != sweeps || run->graphActive != active) {
-                    run->graph = c.captureSlices(run->ptrs, static_cast<uint64_t>(sweeps), k.gridCap);
+                    run->graph = c.captureSlices(run->ptrs, static_cast<uint64_t>(sweeps), k.gridCap, k.threads);
                     run->graphSlice = sweeps;
                     run->graphActive = active;
                 }

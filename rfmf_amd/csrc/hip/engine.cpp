@@ -122,13 +122,40 @@ const DevGraph* Context::graph(int32_t id) const {
     return it->second.get();
 }
 
-int Context::occupancy(int bits, bool mul24, size_t lds) {
-    const uint64_t key = (static_cast<uint64_t>(bits) << 40) | (static_cast<uint64_t>(mul24) << 39) | lds;
+int Context::occupancy(int bits, bool mul24, int threads, size_t lds) {
+    const uint64_t key = (static_cast<uint64_t>(bits) << 48) | (static_cast<uint64_t>(mul24) << 47) |
+                         (static_cast<uint64_t>(threads) << 32) | lds;
     const auto it = occCache_.find(key);
     if (it != occCache_.end()) return it->second;
-    const int occ = launch::maxActiveBlocks(bits, mul24, lds);
+    const int occ = launch::maxActiveBlocks(bits, mul24, threads, lds);
     occCache_[key] = occ;
     return occ;
+}
+
+uint64_t Context::tuneKey(int bits, int64_t maxN, int64_t chains, int maxP) {
+    auto lg = [](int64_t x) { int l = 0; while ((int64_t{1} << (l + 1)) <= x) ++l; return static_cast<uint64_t>(l); };
+    return (static_cast<uint64_t>(bits) << 40) | (lg(std::max<int64_t>(maxN, 1)) << 32) |
+           (lg(std::max<int64_t>(chains, 1)) << 24) | static_cast<uint64_t>(maxP);
+}
+
+bool Context::fitsLds(int64_t maxN, int bits, int maxP, int threads) const {
+    const dev::LdsPlan p = dev::makeLdsPlan(maxN, bits, maxP, threads);
+    return static_cast<size_t>(p.total) + dev::kStaticLds <= props_.sharedMemPerBlock;
+}
+
+int Context::chooseThreads(int bits, int64_t maxN, int64_t chains, int maxP) const {
+    const auto it = tuned_.find(tuneKey(bits, maxN, chains, maxP));
+    if (it != tuned_.end()) return it->second;
+    // Heuristic: with several chains per CU the 4-wave tile keeps every SIMD
+    // busy across chains; with fewer chains than CUs, more waves per chain
+    // cut the per-sweep latency instead.
+    const double perCu = static_cast<double>(chains) / std::max(1, props_.multiProcessorCount);
+    int t = 256;
+    if (perCu >= 2.0) t = maxN <= 2048 ? 128 : 256;
+    else if (perCu >= 1.0) t = 512;
+    else t = maxN >= 4096 ? 1024 : 512;
+    while (t > 128 && !fitsLds(maxN, bits, maxP, t)) t /= 2;
+    return t;
 }
 
 std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t stream) {
@@ -208,7 +235,9 @@ std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t str
         throw Error(Status::Unsupported, "max weighted degree >= 2^30 is not supported");
     // 24-bit multiplies need |beta| < 2^23 and a field below 2^23 (8/16-bit words).
     job->mul24 = betaMax < (int64_t{1} << 23) && job->bits <= 16;
-    const dev::LdsPlan plan = dev::makeLdsPlan(n, job->bits, maxP);
+    // The smallest tile (128 threads) has the smallest plan; launches pick a
+    // tile that fits.
+    const dev::LdsPlan plan = dev::makeLdsPlan(n, job->bits, maxP, 128);
     if (static_cast<size_t>(plan.total) + dev::kStaticLds > props_.sharedMemPerBlock)
         throw Error(Status::Unsupported, "chain state of n=" + std::to_string(n) + " (" +
                                              std::to_string(plan.total) +
@@ -303,7 +332,8 @@ float Context::runSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
     return waitSlice(h);
 }
 
-std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<ResidentJob*>& jobs, int gridCap) {
+std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<ResidentJob*>& jobs, int gridCap,
+                                                      int threads) {
     // Group unfinished chains by kernel variant; items sorted by descending n
     // so the persistent workgroups take the longest chains first.
     std::vector<LaunchGroup> groups(6);
@@ -327,14 +357,22 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
                          [](const dev::ChainItem& a, const dev::ChainItem& b) { return a.n > b.n; });
         g.bits = (gi / 2 == 0) ? 8 : (gi / 2 == 1) ? 16 : 32;
         g.mul24 = (gi % 2) == 1;
-        g.plan = dev::makeLdsPlan(maxN[gi], g.bits, maxP[gi]);
+        g.threads = threads > 0 ? threads
+                                : chooseThreads(g.bits, maxN[gi], static_cast<int64_t>(g.items.size()), maxP[gi]);
+        if (!dev::validThreads(g.threads))
+            throw Error(Status::InvalidArg, "threads per chain must be 128, 256, 512 or 1024");
+        g.plan = dev::makeLdsPlan(maxN[gi], g.bits, maxP[gi], g.threads);
         if (static_cast<size_t>(g.plan.total) + dev::kStaticLds > props_.sharedMemPerBlock)
             throw Error(Status::Unsupported, "combined launch needs " + std::to_string(g.plan.total) +
                                                  " B of LDS per workgroup");
-        const int occ = std::max(1, occupancy(g.bits, g.mul24, g.plan.total));
+        const int occ = std::max(1, occupancy(g.bits, g.mul24, g.threads, g.plan.total));
         int grid = occ * props_.multiProcessorCount;
         if (gridCap > 0) grid = std::min(grid, gridCap);
         g.grid = static_cast<int>(std::min<int64_t>(grid, static_cast<int64_t>(g.items.size())));
+        last_.threads = g.threads;
+        last_.grid = g.grid;
+        last_.bits = g.bits;
+        last_.occupancy = occ;
         out.push_back(std::move(g));
     }
     return out;
@@ -344,7 +382,7 @@ Context::SliceHandle Context::launchSlice(const std::vector<ResidentJob*>& jobs,
     hipCheck(hipSetDevice(device_), "hipSetDevice");
     SliceHandle h;
     h.jobs = jobs;
-    std::vector<LaunchGroup> groups = planGroups(jobs, knobs.gridCap);
+    std::vector<LaunchGroup> groups = planGroups(jobs, knobs.gridCap, knobs.threads);
     const uint64_t slice = knobs.sliceSweeps > 0 ? static_cast<uint64_t>(knobs.sliceSweeps) : ~uint64_t{0};
     h.slice = slice;
     hipCheck(hipEventCreate(&h.ev0), "hipEventCreate");
@@ -355,7 +393,7 @@ Context::SliceHandle Context::launchSlice(const std::vector<ResidentJob*>& jobs,
         h.itemBufs.emplace_back(g.items.size() * sizeof(dev::ChainItem));
         hipCheck(hipMemcpyAsync(h.itemBufs.back().get(), g.items.data(), h.itemBufs.back().bytes(),
                                 hipMemcpyHostToDevice, stream_), "upload items");
-        hipCheck(launch::anneal(g.bits, g.mul24, h.itemBufs.back().as<dev::ChainItem>(),
+        hipCheck(launch::anneal(g.bits, g.mul24, g.threads, h.itemBufs.back().as<dev::ChainItem>(),
                                 static_cast<int>(g.items.size()), sched_.as<uint32_t>() + 2 * slot, g.plan,
                                 g.grid, slice, stream_), "annealSlice launch");
         ++slot;
@@ -371,25 +409,26 @@ Context::GraphedSlices::~GraphedSlices() {
 }
 
 std::unique_ptr<Context::GraphedSlices> Context::captureSlices(const std::vector<ResidentJob*>& jobs,
-                                                               uint64_t sliceSweeps, int gridCap) {
+                                                               uint64_t sliceSweeps, int gridCap,
+                                                               int threads) {
     hipCheck(hipSetDevice(device_), "hipSetDevice");
     auto g = std::make_unique<GraphedSlices>();
     g->jobs = jobs;
     g->slice = sliceSweeps;
-    std::vector<LaunchGroup> groups = planGroups(jobs, gridCap);
+    std::vector<LaunchGroup> groups = planGroups(jobs, gridCap, threads);
     // Item arrays go up before the capture; the graph holds kernel nodes only.
     for (LaunchGroup& lg : groups) {
         g->itemBufs.emplace_back(lg.items.size() * sizeof(dev::ChainItem));
         hipCheck(hipMemcpy(g->itemBufs.back().get(), lg.items.data(), g->itemBufs.back().bytes(),
                            hipMemcpyHostToDevice), "upload items");
-        (void)occupancy(lg.bits, lg.mul24, lg.plan.total);   // sets the LDS attribute outside the capture
+        (void)launch::maxActiveBlocks(lg.bits, lg.mul24, lg.threads, lg.plan.total);   // LDS attribute, outside the capture
     }
     hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     int slot = 0;
     hipError_t err = hipSuccess;
     for (size_t i = 0; i < groups.size() && err == hipSuccess; ++i, ++slot)
-        err = launch::anneal(groups[i].bits, groups[i].mul24, g->itemBufs[i].as<dev::ChainItem>(),
+        err = launch::anneal(groups[i].bits, groups[i].mul24, groups[i].threads, g->itemBufs[i].as<dev::ChainItem>(),
                              static_cast<int>(groups[i].items.size()), sched_.as<uint32_t>() + 2 * slot,
                              groups[i].plan, groups[i].grid, sliceSweeps, stream_);
     const hipError_t endErr = hipStreamEndCapture(stream_, &g->graph);
@@ -434,8 +473,52 @@ float Context::waitSlice(SliceHandle& h) {
     return ms;
 }
 
-float Context::runToCompletion(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs) {
+int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* trialMs) {
+    // Shape of the launch: widest word, largest n and P, unfinished chains.
+    int bits = 8, maxP = 1;
+    int64_t maxN = 0, chains = 0;
+    uint64_t minLeft = ~uint64_t{0};
+    for (const ResidentJob* j : jobs)
+        for (size_t c = 0; c < j->items.size(); ++c) {
+            if (j->stepsDone[c] >= j->totalSteps) continue;
+            bits = std::max(bits, j->bits);
+            maxN = std::max<int64_t>(maxN, j->graph->n);
+            maxP = std::max(maxP, j->maxP);
+            ++chains;
+            minLeft = std::min(minLeft, j->totalSteps - j->stepsDone[c]);
+        }
+    if (chains == 0) return 0;
+    const uint64_t key = tuneKey(bits, maxN, chains, maxP);
+    const auto hit = tuned_.find(key);
+    last_.tuned = true;
+    if (hit != tuned_.end()) return hit->second;
+    // Each candidate runs one short slice of the live run: geometry does not
+    // change any chain's arithmetic, so the trial sweeps are part of the
+    // result, and the trials together use at most a fifth of the sweeps.
+    const uint64_t trial = std::max<uint64_t>(1, std::min<uint64_t>(256, minLeft / 20));
+    int best = 0;
+    float bestPerSweep = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        const int t = dev::kThreadChoices[i];
+        if (!fitsLds(maxN, bits, maxP, t)) continue;
+        LaunchKnobs k = knobs;
+        k.threads = t;
+        k.sliceSweeps = static_cast<int64_t>(trial);
+        const float ms = runSlice(jobs, k);
+        const float per = ms / static_cast<float>(trial);
+        if (trialMs) trialMs[i] = per;
+        last_.trialMsPerSweep[i] = per;
+        if (best == 0 || per < bestPerSweep) { best = t; bestPerSweep = per; }
+    }
+    tuned_[key] = best;
+    return best;
+}
+
+float Context::runToCompletion(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobsIn) {
     float total = 0.0f;
+    LaunchKnobs knobs = knobsIn;
+    last_ = LaunchReport{};
+    if (knobs.autotune && knobs.threads == 0) knobs.threads = autotune(jobs, knobs, nullptr);
     if (knobs.hipGraph && knobs.sliceSweeps > 0) {
         // Every chain starts together here, so ceil(longest / slice) replays
         // of one captured slice finish them all.
@@ -443,7 +526,7 @@ float Context::runToCompletion(const std::vector<ResidentJob*>& jobs, const Laun
         for (const ResidentJob* j : jobs)
             for (uint64_t s : j->stepsDone) longest = std::max(longest, j->totalSteps - s);
         if (longest > 0) {
-            auto g = captureSlices(jobs, static_cast<uint64_t>(knobs.sliceSweeps), knobs.gridCap);
+            auto g = captureSlices(jobs, static_cast<uint64_t>(knobs.sliceSweeps), knobs.gridCap, knobs.threads);
             const uint64_t reps = (longest + knobs.sliceSweeps - 1) / knobs.sliceSweeps;
             total += replaySlices(*g, static_cast<int>(reps));
         }

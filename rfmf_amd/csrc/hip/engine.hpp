@@ -90,8 +90,20 @@ struct ResidentJob {
 
 struct LaunchKnobs {
     int64_t sliceSweeps = 0;   // 0: run to completion in one launch
-    int gridCap = 0;
+    int gridCap = 0;           // 0: occupancy x CUs
     bool hipGraph = false;
+    int threads = 0;           // threads per chain (128/256/512/1024); 0: tuned or heuristic
+    bool autotune = false;     // time the candidates on the run's first slices
+};
+
+// What the last launch of a context used (for reports).
+struct LaunchReport {
+    int threads = 0;
+    int grid = 0;
+    int bits = 0;
+    int occupancy = 0;
+    bool tuned = false;
+    float trialMsPerSweep[4] = {0, 0, 0, 0};   // by kThreadChoices, 0 if not tried
 };
 
 class Context {
@@ -141,14 +153,21 @@ public:
         ~GraphedSlices();
     };
     std::unique_ptr<GraphedSlices> captureSlices(const std::vector<ResidentJob*>& jobs, uint64_t sliceSweeps,
-                                                 int gridCap);
+                                                 int gridCap, int threads);
+    // Autotuner: times each thread-count candidate on one short slice of the
+    // live run (output-invariant), caches and returns the fastest.
+    int autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* trialMs);
     // Replays the captured slice `times` times; returns kernel-side ms.
     float replaySlices(GraphedSlices& g, int times);
     // Copies a finished job's results to the spec's outputs.
     void collect(const ResidentJob& job, const JobSpec& spec, hipStream_t stream);
 
-    // Autotuned knobs cache (by word bits and LDS bytes bucket).
-    int occupancy(int bits, bool mul24, size_t lds);
+    // Resident workgroups per CU of a kernel variant (cached).
+    int occupancy(int bits, bool mul24, int threads, size_t lds);
+    // Threads per chain for a launch shape: the autotuner's cached choice,
+    // else a heuristic from chains per CU and n.
+    int chooseThreads(int bits, int64_t maxN, int64_t chains, int maxP) const;
+    const LaunchReport& lastLaunch() const { return last_; }
 
     std::mutex& mutex() { return mu_; }
 
@@ -157,10 +176,15 @@ private:
         std::vector<dev::ChainItem> items;
         int bits = 8;
         bool mul24 = false;
+        int threads = dev::kThreads;
         dev::LdsPlan plan{};
         int grid = 1;
     };
-    std::vector<LaunchGroup> planGroups(const std::vector<ResidentJob*>& jobs, int gridCap);
+    std::vector<LaunchGroup> planGroups(const std::vector<ResidentJob*>& jobs, int gridCap, int threads);
+    static uint64_t tuneKey(int bits, int64_t maxN, int64_t chains, int maxP);
+    bool fitsLds(int64_t maxN, int bits, int maxP, int threads) const;
+    std::unordered_map<uint64_t, int> tuned_;
+    LaunchReport last_;
 
     int device_;
     hipStream_t stream_ = nullptr;

@@ -82,6 +82,22 @@ def test_slicing_and_graph_replay_invariance(graphs, slice_sweeps, use_graph):
         same(j, one_shot(j), f"slice {slice_sweeps} {k}")
 
 
+@pytest.mark.parametrize("threads,autotune", [(128, False), (512, False), (1024, False), (0, True)])
+def test_threads_per_chain_and_autotune_invariance(graphs, threads, autotune):
+    print(f"[batch] threads={threads} autotune={autotune}", flush=True)
+    jobs = _mixed_jobs(graphs)
+    with R.Context(0) as ctx:
+        ids = {k: ctx.upload_graph(g) for k, g in graphs.items()}
+        ctx.batch_run([(ids[k], j) for k, j in jobs], slice_sweeps=45, threads=threads, autotune=autotune)
+        geo = ctx.last_launch()
+    if threads:
+        assert geo["threads_per_chain"] == threads
+    else:
+        assert len(geo["trial_ms_per_sweep"]) >= 3
+    for k, j in jobs:
+        same(j, one_shot(j), f"threads {threads} {k}")
+
+
 def test_chain_isolation_beyond_resident_capacity(gset):
     # 3000 chains of an 800-vertex graph exceed the resident workgroups of
     # 256 CUs, so the persistent workgroups take several chains each.

@@ -257,7 +257,7 @@ typedef struct rfmf_server rfmf_server;
 typedef struct rfmf_server_stats {
     int64_t submitted;
     int64_t completed;
-    int64_t rejected;
+    int64_t rejected;          /* refused at submit: over budget or invalid (counted in submitted) */
     int64_t active_chains;
     int64_t resident_bytes;
     int64_t slices;
@@ -265,14 +265,18 @@ typedef struct rfmf_server_stats {
     double busy_kernel_ms;     /* kernel-only time of all slices */
 } rfmf_server_stats;
 
-/* A server owns a worker thread that runs slices of `slice_sweeps` sweeps
- * over every admitted chain and admits queued requests between slices while
- * their HBM footprint fits `hbm_budget_bytes`. */
+/* A server owns a worker thread that keeps two slices of `slice_sweeps`
+ * sweeps over every admitted chain queued on the device, admits queued
+ * requests (FIFO) while their HBM footprint fits `hbm_budget_bytes`, and
+ * retires finished requests while the next slice runs. */
 rfmf_status rfmf_server_create(rfmf_ctx* ctx, int64_t hbm_budget_bytes, int64_t slice_sweeps,
                                rfmf_server** out);
 /* Queues a request (graph copied, inputs copied); outputs are written into
  * the job's output pointers when the request completes (they must stay valid
- * until then).  job->graph_id is ignored. */
+ * until then).  job->graph_id is ignored.  The request is validated and its
+ * host state prepared (CSR checks, int32 gate, initial cuts) on the calling
+ * thread, so RFMF_ERR_ADMISSION (footprint over budget), RFMF_ERR_UNSUPPORTED
+ * and RFMF_ERR_INVALID_ARG come back from this call. */
 rfmf_status rfmf_server_submit(rfmf_server* srv, const rfmf_csr* g, const rfmf_job* job,
                                int64_t* request_id);
 /* done = 1 when the request's outputs are written.  Times are seconds since

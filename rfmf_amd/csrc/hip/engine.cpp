@@ -30,10 +30,35 @@ void DevBuf::alloc(size_t bytes) {
     bytes_ = bytes;
 }
 
+void DevBuf::allocAsync(size_t bytes, hipMemPool_t pool, hipStream_t stream) {
+    reset();
+    if (bytes == 0) bytes = 16;
+    hipCheck(hipMallocFromPoolAsync(&p_, bytes, pool, stream), "hipMallocFromPoolAsync");
+    bytes_ = bytes;
+    async_ = true;
+    stream_ = stream;
+}
+
 void DevBuf::reset() {
-    if (p_) (void)hipFree(p_);
+    if (p_) (void)(async_ ? hipFreeAsync(p_, stream_) : hipFree(p_));
     p_ = nullptr;
     bytes_ = 0;
+    async_ = false;
+    stream_ = nullptr;
+}
+
+PinnedBuf::~PinnedBuf() {
+    if (p_) (void)hipHostFree(p_);
+}
+
+void PinnedBuf::reserve(size_t bytes) {
+    if (bytes <= cap_) return;
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+    cap_ = 0;
+    const size_t want = std::max<size_t>(bytes + bytes / 2, 64 << 10);
+    hipCheck(hipHostMalloc(&p_, want, hipHostMallocDefault), "hipHostMalloc");
+    cap_ = want;
 }
 
 int wordBitsFor(int64_t B) {
@@ -63,24 +88,42 @@ Context::~Context() {
     if (side_) (void)hipStreamSynchronize(side_);
     graphs_.clear();
     sched_.reset();
+    for (ItemSlot& s : slots_) {
+        if (s.done) (void)hipEventDestroy(s.done);
+        s.dev.reset();
+    }
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (side_) (void)hipStreamSynchronize(side_);
+    if (pool_) (void)hipMemPoolDestroy(pool_);
     if (stream_) (void)hipStreamDestroy(stream_);
     if (side_) (void)hipStreamDestroy(side_);
 }
 
-int32_t Context::uploadGraph(const uint32_t* offsets, const int32_t* to, const int32_t* w,
-                             int64_t n, int64_t m2) {
+hipMemPool_t Context::pool() {
+    if (pool_) return pool_;
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device_;
+    hipCheck(hipMemPoolCreate(&pool_, &props), "hipMemPoolCreate");
+    // Keep freed memory in the pool: a serving loop admits and retires jobs
+    // all the time, and returning memory to the driver costs a device sync.
+    uint64_t keep = ~uint64_t{0};
+    hipCheck(hipMemPoolSetAttribute(pool_, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute");
+    return pool_;
+}
+
+HostGraph prepareGraph(const uint32_t* offsets, const int32_t* to, const int32_t* w, int64_t n, int64_t m2) {
     if (!offsets || n < 0 || m2 < 0 || (m2 > 0 && (!to || !w)))
         throw Error(Status::InvalidArg, "uploadGraph: null CSR array or negative size");
     if (n >= (int64_t{1} << 31) || m2 >= (int64_t{1} << 32))
         throw Error(Status::Unsupported, "uploadGraph: graph exceeds 32-bit ids/offsets");
     if (offsets[0] != 0 || static_cast<int64_t>(offsets[n]) != m2)
         throw Error(Status::InvalidArg, "uploadGraph: offsets[0] must be 0 and offsets[n] == m2");
-    auto g = std::make_unique<DevGraph>();
-    g->n = n;
-    g->m2 = m2;
-    g->hOffsets.assign(offsets, offsets + n + 1);
-    g->hTo.assign(to, to + m2);
-    g->hW.assign(w, w + m2);
+    HostGraph g;
+    g.n = n;
+    g.m2 = m2;
     int64_t B = 0;
     for (int64_t v = 0; v < n; ++v) {
         if (offsets[v + 1] < offsets[v]) throw Error(Status::InvalidArg, "uploadGraph: offsets not monotone");
@@ -91,28 +134,60 @@ int32_t Context::uploadGraph(const uint32_t* offsets, const int32_t* to, const i
         }
         B = std::max(B, s);
     }
-    g->B = B;
-    std::vector<int32_t> inter(static_cast<size_t>(2 * m2));
+    g.B = B;
+    g.offsets.assign(offsets, offsets + n + 1);
+    g.to.assign(to, to + m2);
+    g.w.assign(w, w + m2);
+    g.edges.resize(static_cast<size_t>(2 * m2));
     for (int64_t i = 0; i < m2; ++i) {
-        inter[2 * i] = to[i];
-        inter[2 * i + 1] = w[i];
+        g.edges[2 * i] = to[i];
+        g.edges[2 * i + 1] = w[i];
     }
+    return g;
+}
+
+int32_t Context::uploadGraph(const uint32_t* offsets, const int32_t* to, const int32_t* w,
+                             int64_t n, int64_t m2, hipStream_t async) {
+    return uploadGraph(prepareGraph(offsets, to, w, n, m2), async);
+}
+
+int32_t Context::uploadGraph(HostGraph&& hg, hipStream_t async) {
+    auto g = std::make_unique<DevGraph>();
+    g->n = hg.n;
+    g->m2 = hg.m2;
+    g->B = hg.B;
+    g->host = std::move(hg);
+    const int64_t n = g->n, m2 = g->m2;
     hipCheck(hipSetDevice(device_), "hipSetDevice");
-    g->offsets.alloc(static_cast<size_t>(n + 1) * sizeof(uint32_t));
-    g->edges.alloc(static_cast<size_t>(std::max<int64_t>(m2, 1)) * 8);
-    hipCheck(hipMemcpy(g->offsets.get(), offsets, (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice),
-             "upload offsets");
-    if (m2 > 0)
-        hipCheck(hipMemcpy(g->edges.get(), inter.data(), m2 * 8, hipMemcpyHostToDevice), "upload edges");
+    const size_t offBytes = static_cast<size_t>(n + 1) * sizeof(uint32_t);
+    const size_t edgeBytes = static_cast<size_t>(std::max<int64_t>(m2, 1)) * 8;
+    if (async) {
+        g->offsets.allocAsync(offBytes, pool(), async);
+        g->edges.allocAsync(edgeBytes, pool(), async);
+        hipCheck(hipMemcpyAsync(g->offsets.get(), g->host.offsets.data(), offBytes, hipMemcpyHostToDevice, async),
+                 "upload offsets");
+        if (m2 > 0)
+            hipCheck(hipMemcpyAsync(g->edges.get(), g->host.edges.data(), m2 * 8, hipMemcpyHostToDevice, async),
+                     "upload edges");
+        hipCheck(hipStreamSynchronize(async), "upload graph");
+    } else {
+        g->offsets.alloc(offBytes);
+        g->edges.alloc(edgeBytes);
+        hipCheck(hipMemcpy(g->offsets.get(), g->host.offsets.data(), offBytes, hipMemcpyHostToDevice),
+                 "upload offsets");
+        if (m2 > 0)
+            hipCheck(hipMemcpy(g->edges.get(), g->host.edges.data(), m2 * 8, hipMemcpyHostToDevice), "upload edges");
+    }
+    std::vector<int32_t>().swap(g->host.edges);
     const int32_t id = nextGraph_++;
     graphs_.emplace(id, std::move(g));
     return id;
 }
 
-void Context::releaseGraph(int32_t id) {
+void Context::releaseGraph(int32_t id, bool wait) {
     const auto it = graphs_.find(id);
     if (it == graphs_.end()) throw Error(Status::NotFound, "unknown graph id " + std::to_string(id));
-    hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    if (wait) hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     graphs_.erase(it);
 }
 
@@ -158,26 +233,31 @@ int Context::chooseThreads(int bits, int64_t maxN, int64_t chains, int maxP) con
     return t;
 }
 
-std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t stream) {
-    const DevGraph* g = spec.graph;
-    if (!g) throw Error(Status::InvalidArg, "job has no graph");
+std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t stream, bool streamOrdered) {
+    if (!spec.graph) throw Error(Status::InvalidArg, "job has no graph");
+    return placeJob(prepareJob(spec, spec.graph->host), spec.graph, stream, streamOrdered);
+}
+
+HostJob Context::prepareJob(const JobSpec& spec, const HostGraph& g) const {
     if (spec.chains <= 0) throw Error(Status::InvalidArg, "job needs at least one chain");
     if (!spec.initialSpins || !spec.seeds || !spec.runEnd || !spec.runVal || !spec.chainRunOff ||
         spec.nRuns < 1)
         throw Error(Status::InvalidArg, "job is missing spins, seeds or beta tables");
     if (spec.phases.size() != spec.iters.size())
         throw Error(Status::InvalidArg, "phases and iters differ in length");
-    const int64_t n = g->n;
+    const int64_t n = g.n;
     const int64_t C = spec.chains;
-
-    auto job = std::make_unique<ResidentJob>();
-    job->graph = g;
-    job->chains = C;
+    HostJob hj;
+    hj.chains = C;
+    hj.n = n;
+    hj.initialSpins = spec.initialSpins;
+    hj.seeds = spec.seeds;
+    hj.runEnd = spec.runEnd;
+    hj.runVal = spec.runVal;
+    hj.nRuns = spec.nRuns;
 
     // Executed phases: P > 0 and at least one sweep (a P == 0 phase is
     // skipped without advancing the sweep counter, src/anneal_kernels.cu:270-272).
-    std::vector<uint64_t> hPhaseEnd;
-    std::vector<uint32_t> hPhaseP;
     uint64_t total = 0;
     int maxP = 1;
     for (size_t p = 0; p < spec.phases.size(); ++p) {
@@ -189,23 +269,23 @@ std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t str
         total += spec.iters[p];
         if (total > static_cast<uint64_t>(kMaxTotalSteps))
             throw Error(Status::StepCap, "sweeps per chain exceed the total-sweep cap");
-        hPhaseEnd.push_back(total);
-        hPhaseP.push_back(static_cast<uint32_t>(P));
+        hj.phaseEnd.push_back(total);
+        hj.phaseP.push_back(static_cast<uint32_t>(P));
         maxP = std::max<int>(maxP, static_cast<int>(P));
     }
-    if (hPhaseEnd.empty()) {   // nothing executes; keep one dummy phase for the item layout
-        hPhaseEnd.push_back(0);
-        hPhaseP.push_back(1);
+    if (hj.phaseEnd.empty()) {   // nothing executes; keep one dummy phase for the item layout
+        hj.phaseEnd.push_back(0);
+        hj.phaseP.push_back(1);
     }
-    job->totalSteps = total;
-    job->maxP = maxP;
+    hj.totalSteps = total;
+    hj.maxP = maxP;
 
     // Per-chain table extents: a table runs from its offset to the next
     // larger distinct offset (tables are concatenated).
-    std::vector<int64_t> starts(spec.chainRunOff, spec.chainRunOff + C);
-    for (int64_t o : starts)
+    hj.tableStart.assign(spec.chainRunOff, spec.chainRunOff + C);
+    for (int64_t o : hj.tableStart)
         if (o < 0 || o >= spec.nRuns) throw Error(Status::InvalidArg, "chain_run_off out of range");
-    std::vector<int64_t> uniq = starts;
+    std::vector<int64_t> uniq = hj.tableStart;
     std::sort(uniq.begin(), uniq.end());
     uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
     auto tableLen = [&](int64_t off) {
@@ -215,6 +295,7 @@ std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t str
     int64_t betaMax = 0;
     for (int64_t off : uniq) {
         const int64_t len = tableLen(off);
+        if (len > INT32_MAX) throw Error(Status::Unsupported, "a beta table has more than 2^31 runs");
         for (int64_t r = off; r < off + len; ++r) {
             betaMax = std::max<int64_t>(betaMax, std::llabs(static_cast<long long>(spec.runVal[r])));
             if (r > off && spec.runEnd[r] < spec.runEnd[r - 1])
@@ -223,21 +304,23 @@ std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t str
         if (static_cast<uint64_t>(spec.runEnd[off + len - 1]) < total)
             throw Error(Status::InvalidArg, "a beta table ends before the last sweep");
     }
+    hj.tableLen.resize(static_cast<size_t>(C));
+    for (int64_t c = 0; c < C; ++c) hj.tableLen[c] = static_cast<int32_t>(tableLen(hj.tableStart[c]));
     if (spec.gateBetaMax >= 0) betaMax = spec.gateBetaMax;
 
-    const int64_t gateB = std::max<int64_t>(spec.gateB, g->B);
+    const int64_t gateB = std::max<int64_t>(spec.gateB, g.B);
     if (!scoreSafeInt32ForBeta(betaMax, gateB))
         throw Error(Status::Unsupported,
                     "the int32 score gate fails (beta_max*B + |gumbel|max >= INT32_MAX); the int64 "
                     "two-pass score path is not implemented");
-    job->bits = wordBitsFor(g->B);
-    if (job->bits == 0)
+    hj.bits = wordBitsFor(g.B);
+    if (hj.bits == 0)
         throw Error(Status::Unsupported, "max weighted degree >= 2^30 is not supported");
     // 24-bit multiplies need |beta| < 2^23 and a field below 2^23 (8/16-bit words).
-    job->mul24 = betaMax < (int64_t{1} << 23) && job->bits <= 16;
+    hj.mul24 = betaMax < (int64_t{1} << 23) && hj.bits <= 16;
     // The smallest tile (128 threads) has the smallest plan; launches pick a
     // tile that fits.
-    const dev::LdsPlan plan = dev::makeLdsPlan(n, job->bits, maxP, 128);
+    const dev::LdsPlan plan = dev::makeLdsPlan(n, hj.bits, maxP, 128);
     if (static_cast<size_t>(plan.total) + dev::kStaticLds > props_.sharedMemPerBlock)
         throw Error(Status::Unsupported, "chain state of n=" + std::to_string(n) + " (" +
                                              std::to_string(plan.total) +
@@ -247,42 +330,64 @@ std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t str
         if (spec.initialSpins[i] != 1 && spec.initialSpins[i] != -1)
             throw Error(Status::InvalidArg, "initial spins must be +1 or -1");
 
+    // Initial cut of every chain by the host recompute (src/anneal_kernels.cu:754-764).
+    hj.state.resize(static_cast<size_t>(C));
+    for (int64_t c = 0; c < C; ++c) {
+        const int64_t cut = anneallib::solver::cutValueRaw(g.offsets.data(), g.to.data(), g.w.data(), n,
+                                                           spec.initialSpins + c * n);
+        hj.state[c] = dev::ChainState{cut, cut, 0, 0, 0};
+    }
+    return hj;
+}
+
+std::unique_ptr<ResidentJob> Context::placeJob(const HostJob& hj, const DevGraph* g, hipStream_t stream,
+                                               bool streamOrdered) {
+    if (!g || g->n != hj.n) throw Error(Status::InvalidArg, "job prepared for another graph");
+    const int64_t n = hj.n;
+    const int64_t C = hj.chains;
+    auto job = std::make_unique<ResidentJob>();
+    job->graph = g;
+    job->chains = C;
+    job->totalSteps = hj.totalSteps;
+    job->maxP = hj.maxP;
+    job->bits = hj.bits;
+    job->mul24 = hj.mul24;
+
     hipCheck(hipSetDevice(device_), "hipSetDevice");
+    hipMemPool_t pool = streamOrdered ? this->pool() : nullptr;
+    auto alloc = [&](DevBuf& b, size_t bytes) {
+        if (streamOrdered) b.allocAsync(bytes, pool, stream);
+        else b.alloc(bytes);
+    };
     job->wordStride = dev::wordBytes(n, job->bits);
     job->bestStride = dev::bestBytes(n);
-    job->words.alloc(static_cast<size_t>(C * job->wordStride));
-    job->best.alloc(static_cast<size_t>(C * job->bestStride));
-    job->states.alloc(static_cast<size_t>(C) * sizeof(dev::ChainState));
-    job->runEnd.alloc(static_cast<size_t>(spec.nRuns) * sizeof(int64_t));
-    job->runVal.alloc(static_cast<size_t>(spec.nRuns) * sizeof(int32_t));
-    job->phaseEnd.alloc(hPhaseEnd.size() * sizeof(uint64_t));
-    job->phaseP.alloc(hPhaseP.size() * sizeof(uint32_t));
+    alloc(job->words, static_cast<size_t>(C * job->wordStride));
+    alloc(job->best, static_cast<size_t>(C * job->bestStride));
+    alloc(job->states, static_cast<size_t>(C) * sizeof(dev::ChainState));
+    alloc(job->runEnd, static_cast<size_t>(hj.nRuns) * sizeof(int64_t));
+    alloc(job->runVal, static_cast<size_t>(hj.nRuns) * sizeof(int32_t));
+    alloc(job->phaseEnd, hj.phaseEnd.size() * sizeof(uint64_t));
+    alloc(job->phaseP, hj.phaseP.size() * sizeof(uint32_t));
     job->footprint = static_cast<int64_t>(job->words.bytes() + job->best.bytes() + job->states.bytes() +
                                           job->runEnd.bytes() + job->runVal.bytes() +
                                           job->phaseEnd.bytes() + job->phaseP.bytes());
 
-    hipCheck(hipMemcpyAsync(job->runEnd.get(), spec.runEnd, spec.nRuns * sizeof(int64_t),
+    hipCheck(hipMemcpyAsync(job->runEnd.get(), hj.runEnd, hj.nRuns * sizeof(int64_t),
                             hipMemcpyHostToDevice, stream), "upload beta ends");
-    hipCheck(hipMemcpyAsync(job->runVal.get(), spec.runVal, spec.nRuns * sizeof(int32_t),
+    hipCheck(hipMemcpyAsync(job->runVal.get(), hj.runVal, hj.nRuns * sizeof(int32_t),
                             hipMemcpyHostToDevice, stream), "upload beta values");
-    hipCheck(hipMemcpyAsync(job->phaseEnd.get(), hPhaseEnd.data(), hPhaseEnd.size() * sizeof(uint64_t),
+    hipCheck(hipMemcpyAsync(job->phaseEnd.get(), hj.phaseEnd.data(), hj.phaseEnd.size() * sizeof(uint64_t),
                             hipMemcpyHostToDevice, stream), "upload phases");
-    hipCheck(hipMemcpyAsync(job->phaseP.get(), hPhaseP.data(), hPhaseP.size() * sizeof(uint32_t),
+    hipCheck(hipMemcpyAsync(job->phaseP.get(), hj.phaseP.data(), hj.phaseP.size() * sizeof(uint32_t),
                             hipMemcpyHostToDevice, stream), "upload phases");
-
-    // Initial state: words and best bits on the device, the initial cut of
-    // every chain by the host recompute (src/anneal_kernels.cu:754-764).
-    DevBuf spins(static_cast<size_t>(C * n));
-    hipCheck(hipMemcpyAsync(spins.get(), spec.initialSpins, C * n, hipMemcpyHostToDevice, stream),
-             "upload spins");
-    std::vector<dev::ChainState> hState(static_cast<size_t>(C));
-    for (int64_t c = 0; c < C; ++c) {
-        const int64_t cut = anneallib::solver::cutValueRaw(g->hOffsets.data(), g->hTo.data(), g->hW.data(),
-                                                           n, spec.initialSpins + c * n);
-        hState[c] = dev::ChainState{cut, cut, 0, 0, 0};
-    }
-    hipCheck(hipMemcpyAsync(job->states.get(), hState.data(), C * sizeof(dev::ChainState),
+    hipCheck(hipMemcpyAsync(job->states.get(), hj.state.data(), C * sizeof(dev::ChainState),
                             hipMemcpyHostToDevice, stream), "upload chain state");
+
+    // Initial words and best bits from the spins, on the device.
+    DevBuf spins;
+    alloc(spins, static_cast<size_t>(C * n));
+    hipCheck(hipMemcpyAsync(spins.get(), hj.initialSpins, C * n, hipMemcpyHostToDevice, stream),
+             "upload spins");
     std::vector<dev::InitItem> init(static_cast<size_t>(C));
     for (int64_t c = 0; c < C; ++c) {
         init[c].offsets = g->offsets.as<uint32_t>();
@@ -293,7 +398,8 @@ std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t str
         init[c].n = static_cast<int32_t>(n);
         init[c].pad = 0;
     }
-    DevBuf dInit(init.size() * sizeof(dev::InitItem));
+    DevBuf dInit;
+    alloc(dInit, init.size() * sizeof(dev::InitItem));
     hipCheck(hipMemcpyAsync(dInit.get(), init.data(), dInit.bytes(), hipMemcpyHostToDevice, stream),
              "upload init items");
     // Chain c of a grid-y launch: gridDim.y is limited to 65535 per launch.
@@ -314,14 +420,14 @@ std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t str
         it.words = reinterpret_cast<uint32_t*>(job->words.as<uint8_t>() + c * job->wordStride);
         it.bestbits = reinterpret_cast<uint32_t*>(job->best.as<uint8_t>() + c * job->bestStride);
         it.st = job->states.as<dev::ChainState>() + c;
-        it.betaEnd = job->runEnd.as<int64_t>() + starts[c];
-        it.betaVal = job->runVal.as<int32_t>() + starts[c];
-        it.nRuns = static_cast<int32_t>(tableLen(starts[c]));
+        it.betaEnd = job->runEnd.as<int64_t>() + hj.tableStart[c];
+        it.betaVal = job->runVal.as<int32_t>() + hj.tableStart[c];
+        it.nRuns = hj.tableLen[c];
         it.phaseEnd = job->phaseEnd.as<uint64_t>();
         it.phaseP = job->phaseP.as<uint32_t>();
-        it.nPhases = static_cast<int32_t>(hPhaseEnd.size());
-        it.key = spec.seeds[c];
-        it.stepEnd = total;
+        it.nPhases = static_cast<int32_t>(hj.phaseEnd.size());
+        it.key = hj.seeds[c];
+        it.stepEnd = hj.totalSteps;
         it.n = static_cast<int32_t>(n);
     }
     return job;
@@ -381,25 +487,45 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
 Context::SliceHandle Context::launchSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs) {
     hipCheck(hipSetDevice(device_), "hipSetDevice");
     SliceHandle h;
-    h.jobs = jobs;
     std::vector<LaunchGroup> groups = planGroups(jobs, knobs.gridCap, knobs.threads);
     const uint64_t slice = knobs.sliceSweeps > 0 ? static_cast<uint64_t>(knobs.sliceSweeps) : ~uint64_t{0};
-    h.slice = slice;
-    hipCheck(hipEventCreate(&h.ev0), "hipEventCreate");
-    hipCheck(hipEventCreate(&h.ev1), "hipEventCreate");
-    hipCheck(hipEventRecord(h.ev0, stream_), "hipEventRecord");
-    int slot = 0;
-    for (LaunchGroup& g : groups) {
-        h.itemBufs.emplace_back(g.items.size() * sizeof(dev::ChainItem));
-        hipCheck(hipMemcpyAsync(h.itemBufs.back().get(), g.items.data(), h.itemBufs.back().bytes(),
-                                hipMemcpyHostToDevice, stream_), "upload items");
-        hipCheck(launch::anneal(g.bits, g.mul24, g.threads, h.itemBufs.back().as<dev::ChainItem>(),
-                                static_cast<int>(g.items.size()), sched_.as<uint32_t>() + 2 * slot, g.plan,
-                                g.grid, slice, stream_), "annealSlice launch");
-        ++slot;
+    size_t nItems = 0;
+    for (const LaunchGroup& g : groups) nItems += g.items.size();
+    if (nItems > 0) {
+        // The slot's previous slice must be done with its item array before
+        // the staging copy is overwritten (normally long finished).
+        ItemSlot& slot = slots_[nextSlot_];
+        nextSlot_ ^= 1;
+        if (slot.done) hipCheck(hipEventSynchronize(slot.done), "item slot");
+        else hipCheck(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming), "hipEventCreate");
+        const size_t bytes = nItems * sizeof(dev::ChainItem);
+        slot.host.reserve(bytes);
+        if (slot.dev.bytes() < bytes) slot.dev.alloc(slot.host.capacity());
+        auto* staged = static_cast<dev::ChainItem*>(slot.host.get());
+        size_t at = 0;
+        for (const LaunchGroup& g : groups) {
+            std::memcpy(staged + at, g.items.data(), g.items.size() * sizeof(dev::ChainItem));
+            at += g.items.size();
+        }
+        hipCheck(hipEventCreate(&h.ev0), "hipEventCreate");
+        hipCheck(hipEventCreate(&h.ev1), "hipEventCreate");
+        hipCheck(hipEventRecord(h.ev0, stream_), "hipEventRecord");
+        hipCheck(hipMemcpyAsync(slot.dev.get(), staged, bytes, hipMemcpyHostToDevice, stream_), "upload items");
+        at = 0;
+        int ctr = 0;
+        for (LaunchGroup& g : groups) {
+            hipCheck(launch::anneal(g.bits, g.mul24, g.threads, slot.dev.as<dev::ChainItem>() + at,
+                                    static_cast<int>(g.items.size()), sched_.as<uint32_t>() + 2 * ctr, g.plan,
+                                    g.grid, slice, stream_), "annealSlice launch");
+            at += g.items.size();
+            ++ctr;
+        }
+        hipCheck(hipEventRecord(slot.done, stream_), "hipEventRecord");
+        hipCheck(hipEventRecord(h.ev1, stream_), "hipEventRecord");
         h.empty = false;
     }
-    hipCheck(hipEventRecord(h.ev1, stream_), "hipEventRecord");
+    for (ResidentJob* j : jobs)
+        for (auto& s : j->stepsDone) s = (j->totalSteps - s > slice) ? s + slice : j->totalSteps;
     return h;
 }
 
@@ -466,10 +592,6 @@ float Context::waitSlice(SliceHandle& h) {
         (void)hipEventDestroy(h.ev1);
         h.ev0 = h.ev1 = nullptr;
     }
-    h.itemBufs.clear();
-    for (ResidentJob* j : h.jobs)
-        for (auto& s : j->stepsDone) s = (j->totalSteps - s > h.slice) ? s + h.slice : j->totalSteps;
-    h.jobs.clear();
     return ms;
 }
 

@@ -20,7 +20,11 @@ namespace rfmf {
 
 void hipCheck(hipError_t e, const char* what);
 
-// RAII device buffer.
+// RAII device buffer.  alloc() uses hipMalloc and frees with hipFree, which
+// waits for every stream of the device; allocAsync() takes the memory from a
+// stream-ordered pool and gives it back with hipFreeAsync on the same stream,
+// so retiring a buffer does not stall kernels that are still running.  The
+// owner must make sure no queued work on another stream still uses it.
 class DevBuf {
 public:
     DevBuf() = default;
@@ -28,30 +32,67 @@ public:
     ~DevBuf() { reset(); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; o.bytes_ = 0; }
+    DevBuf(DevBuf&& o) noexcept { take(o); }
     DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { reset(); p_ = o.p_; bytes_ = o.bytes_; o.p_ = nullptr; o.bytes_ = 0; }
+        if (this != &o) { reset(); take(o); }
         return *this;
     }
     void alloc(size_t bytes);
+    void allocAsync(size_t bytes, hipMemPool_t pool, hipStream_t stream);
     void reset();
     template <class T> T* as() const { return static_cast<T*>(p_); }
     void* get() const { return p_; }
     size_t bytes() const { return bytes_; }
 
 private:
+    void take(DevBuf& o) {
+        p_ = o.p_; bytes_ = o.bytes_; async_ = o.async_; stream_ = o.stream_;
+        o.p_ = nullptr; o.bytes_ = 0; o.async_ = false; o.stream_ = nullptr;
+    }
     void* p_ = nullptr;
     size_t bytes_ = 0;
+    bool async_ = false;
+    hipStream_t stream_ = nullptr;
 };
+
+// Grow-only pinned host buffer.
+class PinnedBuf {
+public:
+    PinnedBuf() = default;
+    ~PinnedBuf();
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    void reserve(size_t bytes);
+    void* get() const { return p_; }
+    size_t capacity() const { return cap_; }
+
+private:
+    void* p_ = nullptr;
+    size_t cap_ = 0;
+};
+
+// Host side of a graph: the validated CSR (kept for the initial cut), its
+// max weighted degree, and the interleaved {to, weight} array the kernels
+// read (dropped once uploaded).
+struct HostGraph {
+    int64_t n = 0;
+    int64_t m2 = 0;
+    int64_t B = 0;
+    std::vector<uint32_t> offsets;
+    std::vector<int32_t> to, w;
+    std::vector<int32_t> edges;   // int2[m2]
+};
+// Validates a CSR (offsets monotone from 0 to m2, neighbour ids in range)
+// and builds its HostGraph.  Host-only; any thread.
+HostGraph prepareGraph(const uint32_t* offsets, const int32_t* to, const int32_t* w, int64_t n, int64_t m2);
 
 struct DevGraph {
     DevBuf offsets;   // u32[n+1]
     DevBuf edges;     // int2[m2] {to, weight}
     int64_t n = 0;
     int64_t m2 = 0;
-    int64_t B = 0;    // max weighted degree (computed at upload)
-    std::vector<uint32_t> hOffsets;   // host copy for the initial cut
-    std::vector<int32_t> hTo, hW;
+    int64_t B = 0;    // max weighted degree
+    HostGraph host;   // host copy for the initial cut
 };
 
 // Host description of one job (a set of chains of one graph).
@@ -73,6 +114,29 @@ struct JobSpec {
     int64_t* finalVals = nullptr;
 };
 
+// Host side of an admission: everything admit() derives from a JobSpec and
+// its graph before any device work (validation, gate, word width, executed
+// phases, per-chain beta-table extents, initial cut per chain).  The spec's
+// arrays are borrowed and must outlive the placement.
+struct HostJob {
+    int64_t chains = 0;
+    int64_t n = 0;
+    int bits = 32;
+    bool mul24 = false;
+    int maxP = 1;
+    uint64_t totalSteps = 0;
+    std::vector<uint64_t> phaseEnd;
+    std::vector<uint32_t> phaseP;
+    std::vector<int64_t> tableStart;   // per chain
+    std::vector<int32_t> tableLen;     // per chain
+    std::vector<dev::ChainState> state;
+    const int8_t* initialSpins = nullptr;
+    const uint64_t* seeds = nullptr;
+    const int64_t* runEnd = nullptr;
+    const int32_t* runVal = nullptr;
+    int64_t nRuns = 0;
+};
+
 // Chain state of one admitted job, resident in HBM.
 struct ResidentJob {
     const DevGraph* graph = nullptr;
@@ -84,7 +148,7 @@ struct ResidentJob {
     DevBuf words, best, states, runEnd, runVal, phaseEnd, phaseP;
     int64_t wordStride = 0, bestStride = 0;
     std::vector<dev::ChainItem> items;   // one per chain (device pointers filled)
-    std::vector<uint64_t> stepsDone;     // host mirror of each chain's step after the last sync
+    std::vector<uint64_t> stepsDone;     // host mirror of each chain's step after every enqueued slice
     int64_t footprint = 0;
 };
 
@@ -116,25 +180,38 @@ public:
     hipStream_t sideStream() const { return side_; }
     const hipDeviceProp_t& props() const { return props_; }
 
+    // Copies a CSR to the device.  With `async` the buffers come from the
+    // stream-ordered pool on that stream (the call still returns after the
+    // copies have landed) and are freed without a device-wide wait.
     int32_t uploadGraph(const uint32_t* offsets, const int32_t* to, const int32_t* w, int64_t n,
-                        int64_t m2);
-    void releaseGraph(int32_t id);
+                        int64_t m2, hipStream_t async = nullptr);
+    int32_t uploadGraph(HostGraph&& g, hipStream_t async = nullptr);
+    // Frees a graph.  `wait` first drains stream(); a caller that passes false
+    // must know that no queued launch still reads the graph.
+    void releaseGraph(int32_t id, bool wait = true);
     const DevGraph* graph(int32_t id) const;
 
     // Allocates and initialises a job's chain state on `stream` (initial
     // words from the spins, state {cut, cut, 0, 0}).  Validates the job.
-    std::unique_ptr<ResidentJob> admit(const JobSpec& spec, hipStream_t stream);
+    // With `streamOrdered` every buffer comes from the stream-ordered pool on
+    // `stream`, so admitting and retiring jobs never waits for running slices.
+    std::unique_ptr<ResidentJob> admit(const JobSpec& spec, hipStream_t stream, bool streamOrdered = false);
+    // admit() in two halves: the host half (any thread, no device calls) and
+    // the device half (allocation, uploads and the initial-word kernel).
+    HostJob prepareJob(const JobSpec& spec, const HostGraph& g) const;
+    std::unique_ptr<ResidentJob> placeJob(const HostJob& hj, const DevGraph* g, hipStream_t stream,
+                                          bool streamOrdered);
     // Advances every unfinished chain of `jobs` by up to `sliceSweeps` sweeps
     // (all of them when 0), as few launches as there are kernel variants.
     // Returns the kernel-only milliseconds measured with events on stream().
     float runSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs);
     // The same, split: enqueue the launches, then wait for them (the host can
-    // admit new work on the side stream in between).
+    // admit new work on the side stream in between, and may enqueue the next
+    // slice before waiting: launches of one context run in order on stream()).
+    // The host mirror of every chain's step (ResidentJob::stepsDone) advances
+    // when the slice is enqueued.
     struct SliceHandle {
-        std::vector<DevBuf> itemBufs;
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
-        std::vector<ResidentJob*> jobs;
-        uint64_t slice = 0;
         bool empty = true;
     };
     SliceHandle launchSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs);
@@ -192,6 +269,18 @@ private:
     hipDeviceProp_t props_{};
     DevBuf sched_;   // 2 x u32 work counters, zero between launches
     std::unordered_map<int32_t, std::unique_ptr<DevGraph>> graphs_;
+    // Item arrays of enqueued slices: two slots used in turn, each a device
+    // buffer, its pinned staging copy and the event after the launches that
+    // read it (a slot is reused once that event has completed).
+    struct ItemSlot {
+        DevBuf dev;
+        PinnedBuf host;
+        hipEvent_t done = nullptr;
+    };
+    ItemSlot slots_[2];
+    int nextSlot_ = 0;
+    hipMemPool_t pool_ = nullptr;
+    hipMemPool_t pool();
     int32_t nextGraph_ = 1;
     std::unordered_map<uint64_t, int> occCache_;
     std::mutex mu_;

@@ -1,18 +1,23 @@
 // server.cpp -- continuous batching of MAX-CUT solve requests.
 //
-// A worker thread owns the device loop.  Each iteration it enqueues one slice
-// (every admitted chain advances by `slice` sweeps in one persistent launch
-// per kernel variant), and while that slice runs it admits queued requests
-// whose HBM footprint fits the budget: the CSR upload, the chain-state
-// allocation and the initial-word kernel go to the side stream and overlap the
-// running slice.  After the slice it retires finished requests (results
-// copied to the caller's buffers, device state freed) and loops.  A chain's
-// arithmetic does not depend on when it was admitted or how its sweeps were
-// sliced, so every result equals the one-shot result for the same inputs.
+// A worker thread owns the device loop and keeps two slices queued on the
+// context's stream: every admitted chain advances by `slice` sweeps per slice,
+// in one persistent launch per kernel variant.  While the older slice runs,
+// the loop admits queued requests whose HBM footprint fits the budget (CSR
+// upload, chain-state allocation and the initial-word kernel on the side
+// stream, all from a stream-ordered pool, so nothing waits for the running
+// kernels), and once it has finished, retires the requests whose last sweeps
+// it held (results copied to the caller's buffers on the side stream, device
+// state returned to the pool) while the newer slice keeps the device busy.
+// Which requests end in a slice is known when it is enqueued: the host
+// mirrors every chain's step count.  A chain's arithmetic does not depend on
+// when it was admitted or how its sweeps were sliced, so every result equals
+// the one-shot result for the same inputs.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <memory>
+#include <utility>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -30,8 +35,8 @@ namespace {
 
 struct Request {
     int64_t id = 0;
-    std::vector<uint32_t> off;
-    std::vector<int32_t> to, w;
+    rfmf::HostGraph graph;   // prepared on submit, moved to the device at admission
+    rfmf::HostJob host;      // prepared on submit (validation, gate, initial cuts)
     int64_t n = 0, m2 = 0;
     std::vector<int8_t> spins;
     std::vector<uint64_t> seeds;
@@ -86,9 +91,9 @@ struct rfmf_server {
     void admitOne(const std::shared_ptr<Request>& r) {
         try {
             std::lock_guard<std::mutex> lk(ctx->mutex());
-            r->graphId = ctx->uploadGraph(r->off.data(), r->to.data(), r->w.data(), r->n, r->m2);
+            r->graphId = ctx->uploadGraph(std::move(r->graph), ctx->sideStream());
             r->spec.graph = ctx->graph(r->graphId);
-            r->job = ctx->admit(r->spec, ctx->sideStream());
+            r->job = ctx->placeJob(r->host, r->spec.graph, ctx->sideStream(), /*streamOrdered=*/true);
         } catch (const std::exception& e) {
             r->failed = true;
             r->error = e.what();
@@ -102,79 +107,112 @@ struct rfmf_server {
         }
     }
 
-    // Called with mu held.
+    // Called with mu held, once no queued slice reads the request's buffers.
     void finish(const std::shared_ptr<Request>& r) {
         r->tDone = now();
         r->done = true;
-        r->job.reset();
-        if (r->graphId) {
-            try {
-                std::lock_guard<std::mutex> lk(ctx->mutex());
-                ctx->releaseGraph(r->graphId);
-            } catch (...) {
+        {
+            std::lock_guard<std::mutex> lk(ctx->mutex());
+            r->job.reset();   // stream-ordered frees on the side stream
+            if (r->graphId) {
+                try {
+                    ctx->releaseGraph(r->graphId, /*wait=*/false);
+                } catch (...) {
+                }
+                r->graphId = 0;
             }
-            r->graphId = 0;
         }
         resident -= r->footprint;
         stats.completed += 1;
         doneCv.notify_all();
     }
 
+    struct InFlight {
+        rfmf::Context::SliceHandle h;
+        std::vector<std::shared_ptr<Request>> finishing;   // requests whose last sweeps it holds
+    };
+
+    static bool allEnqueued(const Request& r) {
+        for (uint64_t s : r.job->stepsDone)
+            if (s < r.job->totalSteps) return false;
+        return true;
+    }
+
+    // Enqueues one slice over every active request; requests that reach
+    // their last sweep in it leave `active`.
+    void enqueueSlice(std::deque<InFlight>& inflight) {
+        std::vector<rfmf::ResidentJob*> jobs;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (auto& r : active) jobs.push_back(r->job.get());
+        }
+        InFlight f;
+        rfmf::LaunchKnobs knobs;
+        knobs.sliceSweeps = slice;
+        {
+            std::lock_guard<std::mutex> lk(ctx->mutex());
+            f.h = ctx->launchSlice(jobs, knobs);
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        std::vector<std::shared_ptr<Request>> still;
+        for (auto& r : active) (allEnqueued(*r) ? f.finishing : still).push_back(r);
+        active.swap(still);
+        inflight.push_back(std::move(f));
+    }
+
+    // Waits for the oldest queued slice and hands its finished requests back.
+    void retireOldest(std::deque<InFlight>& inflight) {
+        InFlight f = std::move(inflight.front());
+        inflight.pop_front();
+        float ms = 0.0f;
+        {
+            std::lock_guard<std::mutex> lk(ctx->mutex());
+            ms = ctx->waitSlice(f.h);
+        }
+        for (auto& r : f.finishing) {
+            try {
+                std::lock_guard<std::mutex> lk(ctx->mutex());
+                ctx->collect(*r->job, r->spec, ctx->sideStream());
+            } catch (const std::exception& e) {
+                r->failed = true;
+                r->error = e.what();
+            }
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        if (!f.h.empty) stats.slices += 1;
+        stats.busy_kernel_ms += ms;
+        for (auto& r : f.finishing) {
+            stats.chain_steps += static_cast<double>(r->job->totalSteps) * static_cast<double>(r->job->chains);
+            finish(r);
+        }
+    }
+
     void loop() {
+        std::deque<InFlight> inflight;
         for (;;) {
-            std::vector<std::shared_ptr<Request>> admitNow;
-            std::vector<rfmf::ResidentJob*> jobs;
+            bool haveActive = false;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                wake.wait(lk, [&] { return stopping || !queue.empty() || !active.empty(); });
-                if (stopping && queue.empty() && active.empty()) return;
-                if (active.empty()) {
-                    admitNow = takeAdmissible();
-                } else {
-                    for (auto& r : active) jobs.push_back(r->job.get());
-                }
+                if (inflight.empty())
+                    wake.wait(lk, [&] { return stopping || !queue.empty() || !active.empty(); });
+                if (stopping && queue.empty() && active.empty() && inflight.empty()) return;
+                haveActive = !active.empty();
             }
-            if (jobs.empty()) {
-                for (auto& r : admitNow) admitOne(r);
-                continue;
-            }
-            rfmf::Context::SliceHandle h;
-            rfmf::LaunchKnobs knobs;
-            knobs.sliceSweeps = slice;
-            {
-                std::lock_guard<std::mutex> lk(ctx->mutex());
-                h = ctx->launchSlice(jobs, knobs);
-            }
-            // Admit while the slice runs.
+            // Keep two slices queued so the device never waits for the host.
+            if (haveActive && inflight.size() < 2) enqueueSlice(inflight);
+            // Admit while the queued slices run.
+            std::vector<std::shared_ptr<Request>> admitNow;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 admitNow = takeAdmissible();
             }
             for (auto& r : admitNow) admitOne(r);
-            float ms = 0.0f;
             {
-                std::lock_guard<std::mutex> lk(ctx->mutex());
-                ms = ctx->waitSlice(h);
+                std::lock_guard<std::mutex> lk(mu);
+                haveActive = !active.empty();
             }
-            std::lock_guard<std::mutex> lk(mu);
-            stats.slices += 1;
-            stats.busy_kernel_ms += ms;
-            std::vector<std::shared_ptr<Request>> still;
-            for (auto& r : active) {
-                bool fin = true;
-                for (uint64_t s : r->job->stepsDone) fin &= s >= r->job->totalSteps;
-                if (!fin) { still.push_back(r); continue; }
-                stats.chain_steps += static_cast<double>(r->job->totalSteps) * static_cast<double>(r->job->chains);
-                try {
-                    std::lock_guard<std::mutex> lk2(ctx->mutex());
-                    ctx->collect(*r->job, r->spec, ctx->stream());
-                } catch (const std::exception& e) {
-                    r->failed = true;
-                    r->error = e.what();
-                }
-                finish(r);
-            }
-            active.swap(still);
+            if (haveActive && inflight.size() < 2) continue;   // room for another slice
+            if (!inflight.empty()) retireOldest(inflight);
         }
     }
 };
@@ -203,9 +241,6 @@ rfmf_status rfmf_server_submit(rfmf_server* srv, const rfmf_csr* g, const rfmf_j
     auto r = std::make_shared<Request>();
     r->n = g->n;
     r->m2 = g->m2;
-    r->off.assign(g->offsets, g->offsets + g->n + 1);
-    r->to.assign(g->edge_to, g->edge_to + g->m2);
-    r->w.assign(g->edge_weight, g->edge_weight + g->m2);
     r->spins.assign(job->initial_spins, job->initial_spins + job->chains * g->n);
     r->seeds.assign(job->solver_seeds, job->solver_seeds + job->chains);
     r->runEnd.assign(job->beta.run_end, job->beta.run_end + job->beta.n_runs);
@@ -226,13 +261,30 @@ rfmf_status rfmf_server_submit(rfmf_server* srv, const rfmf_csr* g, const rfmf_j
     s.bestVals = job->best_vals;
     s.finalVals = job->final_vals;
     r->footprint = rfmf_footprint_bytes(g->n, g->m2, job->chains, job->beta.n_runs, job->n_phases);
-    std::lock_guard<std::mutex> lk(srv->mu);
-    srv->stats.submitted += 1;
-    if (r->footprint > srv->budget) {
+    auto reject = [&](rfmf_status st, const std::string& why) {
+        rfmf::setLastError(why);
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stats.submitted += 1;
         srv->stats.rejected += 1;
-        return RFMF_ERR_ADMISSION;
+        return st;
+    };
+    if (r->footprint > srv->budget)
+        return reject(RFMF_ERR_ADMISSION, "request footprint " + std::to_string(r->footprint) +
+                                              " B exceeds the server's HBM budget");
+    // The host half of admission runs here, on the caller's thread (CSR and
+    // job validation, the int32 gate, initial cuts): the device loop then
+    // only allocates, uploads and launches.
+    try {
+        r->graph = rfmf::prepareGraph(g->offsets, g->edge_to, g->edge_weight, g->n, g->m2);
+        r->host = srv->ctx->prepareJob(s, r->graph);
+    } catch (const rfmf::Error& e) {
+        return reject(static_cast<rfmf_status>(e.status()), e.what());
+    } catch (const std::exception& e) {
+        return reject(RFMF_ERR_INVALID_ARG, e.what());
     }
+    std::lock_guard<std::mutex> lk(srv->mu);
     if (srv->stopping) return RFMF_ERR_STATE;
+    srv->stats.submitted += 1;
     r->id = srv->nextId++;
     r->tSubmit = srv->now();
     srv->all.emplace(r->id, r);

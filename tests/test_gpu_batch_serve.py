@@ -162,11 +162,17 @@ def test_server_results_equal_one_shot(graphs):
             with pytest.raises(R.RfmfError) as ei:
                 srv.submit(big)
             assert ei.value.status == 10
+            # beta_max * B + |gumbel| max >= 2^31 - 1 (t0 = 0.005, B = 4): the
+            # int32 gate refuses the request at submit, before any device work.
+            gated = make_job(graphs["G11"], [41], [8, 1], [300, 300], linear=False, t0=[0.005])
+            with pytest.raises(R.RfmfError) as ei:
+                srv.submit(gated)
+            assert ei.value.status == 2
             for rid in ids:
                 srv.wait(rid, 120.0)
             lat = [srv.poll(rid) for rid in ids]
             st = srv.stats()
     assert all(d for d, *_ in lat)
-    assert st["completed"] == len(reqs) and st["rejected"] == 1
+    assert st["completed"] == len(reqs) and st["rejected"] == 2
     for (k, j) in reqs:
         same(j, one_shot(j), f"served {k}")

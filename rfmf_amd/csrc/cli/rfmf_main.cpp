@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <iomanip>
+#include <memory>
 #include <iostream>
 #include <random>
 #include <sstream>
@@ -126,6 +127,8 @@ int main(int argc, char** argv) {
         std::cerr << "Error: failed to read graph: " << a.filename << "\n";
         return 1;
     }
+    // The graph owns the arrays `view` points into: it lives until main returns.
+    const std::unique_ptr<rfmf_graph, void (*)(rfmf_graph*)> graphOwner(gh, rfmf_graph_free);
     rfmf_csr view{};
     rfmf_graph_view(gh, &view);
     const int64_t n = view.n;
@@ -133,7 +136,6 @@ int main(int argc, char** argv) {
                              std::vector<int32_t>(view.edge_to, view.edge_to + view.m2),
                              std::vector<int32_t>(view.edge_weight, view.edge_weight + view.m2),
                              std::vector<int64_t>(rfmf_graph_original_ids(gh), rfmf_graph_original_ids(gh) + n));
-    rfmf_graph_free(gh);
     std::cerr << "Loaded: n=" << rs.num_vertices << " m=" << rs.num_edges << " weight-scale=" << rs.scale
               << " max-weighted-degree=" << rs.max_abs_weighted_degree
               << (rs.float_weights ? " (float weights)" : "")

@@ -17,7 +17,9 @@
 //     plus a best-spin bitmask;
 //   * scoring is one v_mad_i32_i24 per vertex: the bias is folded into a
 //     per-sweep copy of the Gumbel LUT (LUT - bias*beta, wrapping mod 2^32;
-//     the final score is exact because it fits int32 under the gate);
+//     the final score is exact because it fits int32 under the gate); when
+//     the gate fails, the same kernel scores in exact int64 and resolves
+//     (score, id) pairs instead of packed 64-bit keys;
 //   * the per-group argmax is kept in registers (each lane's 8 vertex slots
 //     map to fixed groups when P divides 8*threads), folded across lanes with
 //     a butterfly and across the 4 waves through LDS: no atomics on the hot
@@ -157,12 +159,25 @@ __device__ __forceinline__ uint64_t packKey(int32_t score, int32_t v) {
            static_cast<uint64_t>(~static_cast<uint32_t>(v));
 }
 
+// Order-preserving unsigned image of an int64 score.  Real scores satisfy
+// |score| < 2^62 (|beta| < 2^31, |delta| < 2^30), so their image is never 0,
+// which therefore marks an empty slot.
+__device__ __forceinline__ uint64_t biased64(int64_t s) { return static_cast<uint64_t>(s) ^ (1ull << 63); }
+
 // (score, id) argmax with ties to the smaller id; an empty slot has v = -1
-// and score INT32_MIN, which no real score reaches under the int32 gate.
-__device__ __forceinline__ void keepBetter(int32_t& s, int32_t& v, int32_t s2, int32_t v2) {
+// and the type's minimum score, which no real score reaches.
+template <class S>
+__device__ __forceinline__ void keepBetter(S& s, int32_t& v, S s2, int32_t v2) {
     const bool take = (s2 > s) || (s2 == s && static_cast<uint32_t>(v2) < static_cast<uint32_t>(v));
     s = take ? s2 : s;
     v = take ? v2 : v;
+}
+
+__device__ __forceinline__ int32_t shflXor(int32_t x, int m) { return __shfl_xor(x, m, 64); }
+__device__ __forceinline__ int64_t shflXor(int64_t x, int m) {
+    const int32_t lo = __shfl_xor(static_cast<int32_t>(x), m, 64);
+    const int32_t hi = __shfl_xor(static_cast<int32_t>(x >> 32), m, 64);
+    return (static_cast<int64_t>(hi) << 32) | static_cast<uint32_t>(lo);
 }
 
 __device__ __forceinline__ int64_t waveSum64(int64_t x) {
@@ -186,22 +201,45 @@ __device__ __forceinline__ int32_t scoreOf(int32_t beta, uint32_t field, uint32_
         return static_cast<int32_t>(static_cast<uint32_t>(beta) * field + base);
 }
 
-template <int kBits, bool kMul24, int NT>
+// Exact int64 score: beta * field + (LUT - bias*beta) + fine.
+__device__ __forceinline__ int64_t scoreWide(int32_t beta, uint32_t field, int64_t lutBiased, uint32_t fine) {
+    return static_cast<int64_t>(beta) * static_cast<int64_t>(field) + lutBiased + static_cast<int64_t>(fine);
+}
+
+template <int kBits, int kMode, int NT>
 __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ items,
                                                   int nItems, uint32_t* __restrict__ sched,
                                                   LdsPlan plan, uint64_t sliceSweeps) {
     constexpr int NW = NT / 64;
+    constexpr bool kMul24 = kMode == kScore24;
+    constexpr bool kWide = kMode == kScore64;
+    using Score = std::conditional_t<kWide, int64_t, int32_t>;
+    constexpr Score kNoScore = kWide ? static_cast<Score>(INT64_MIN) : static_cast<Score>(INT32_MIN);
     using F = WordFmt<kBits>;
     extern __shared__ __align__(16) unsigned char lds[];
     uint32_t* wl = reinterpret_cast<uint32_t*>(lds + plan.wordsOff);
     uint32_t* bl = reinterpret_cast<uint32_t*>(lds + plan.bestOff);
     uint8_t* bl8 = reinterpret_cast<uint8_t*>(lds + plan.bestOff);
     uint64_t* part = reinterpret_cast<uint64_t*>(lds + plan.partOff);
+    uint32_t* partId = reinterpret_cast<uint32_t*>(lds + plan.partIdOff);   // int64 scores only
     int32_t* pk = reinterpret_cast<int32_t*>(lds + plan.pickOff);
     int32_t* ps = reinterpret_cast<int32_t*>(lds + plan.spinOff);
     int32_t* pd = reinterpret_cast<int32_t*>(lds + plan.deltaOff);
-    // Static LDS, so the table's address folds into the reads' offset field.
-    __shared__ uint32_t lutb[16];
+    // Static LDS, so the table's address folds into the reads' offset field:
+    // 16 x u32 (int32 scores) or 16 x int64 (int64 scores).
+    __shared__ __align__(16) uint32_t lutRaw[32];
+    uint32_t* lutb = lutRaw;
+    int64_t* lutw = reinterpret_cast<int64_t*>(lutRaw);
+    auto setLut = [&](int32_t b) {
+        const int t = threadIdx.x;
+        if (t < 16) {
+            if constexpr (kWide)
+                lutw[t] = static_cast<int64_t>(kGumbelCoarse[t]) - static_cast<int64_t>(F::kBias) * b;
+            else
+                lutb[t] = static_cast<uint32_t>(kGumbelCoarse[t]) -
+                          static_cast<uint32_t>(F::kBias) * static_cast<uint32_t>(b);
+        }
+    };
     int64_t* dpart = reinterpret_cast<int64_t*>(lds + plan.dpartOff);
     int32_t* misc = reinterpret_cast<int32_t*>(lds + plan.miscOff);
 
@@ -251,7 +289,10 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
         const uint64_t itemEnd = uni(it.stepEnd);
         const uint64_t remaining = itemEnd > step ? itemEnd - step : 0;
         const uint64_t stepEnd = step + (remaining < sliceSweeps ? remaining : sliceSweeps);
-        for (int g = tid; g < plan.slotWords; g += NT) part[g] = 0ull;
+        for (int g = tid; g < plan.slotWords; g += NT) {
+            part[g] = 0ull;
+            if constexpr (kWide) partId[g] = 0u;
+        }
 
         // Phase and beta-run cursors for the slice's first sweep.
         int ph = 0;
@@ -269,9 +310,7 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
         }
         uint64_t runStop = static_cast<uint64_t>(uni(betaEnd[run]));
         int32_t beta = uni(betaVal[run]);
-        if (tid < 16)
-            lutb[tid] = static_cast<uint32_t>(kGumbelCoarse[tid]) -
-                        static_cast<uint32_t>(F::kBias) * static_cast<uint32_t>(beta);
+        setLut(beta);
         __syncthreads();
 
         const int nFull = n >> 3;
@@ -290,15 +329,29 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
             const bool laneReduce = pow2 && P <= kMaxLaneReduceP && ((8 * NT) % P) == 0;
 
             // ---------------- SELECT ----------------
+            // Per-wave partial of a group: a packed (score, ~id) key for int32
+            // scores, a biased score plus ~id beside it for int64 scores.
+            auto putPartial = [&](int idx, Score sc, int32_t v) {
+                if constexpr (kWide) {
+                    part[idx] = v >= 0 ? biased64(sc) : 0ull;
+                    partId[idx] = v >= 0 ? ~static_cast<uint32_t>(v) : 0u;
+                } else {
+                    part[idx] = packKey(sc, v);
+                }
+            };
+            // int64 scores with LDS slots take two passes (the max score, then
+            // the smallest id holding it); this flag marks the second.
+            bool idPass = false;
             auto select = [&](auto laneTag) {
                 constexpr bool kLane = decltype(laneTag)::value;
-                int32_t accS[8], accV[8];
+                Score accS[8];
+                int32_t accV[8];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) { accS[j] = INT32_MIN; accV[j] = -1; }
+                for (int j = 0; j < 8; ++j) { accS[j] = kNoScore; accV[j] = -1; }
                 auto scoreBlock = [&](int b, bool tail) {
                     Block8<kBits> blk;
                     blk.load(wl, b);
-                    if (improved) bl8[b] = static_cast<uint8_t>(blk.spinByte());
+                    if (improved && !idPass) bl8[b] = static_cast<uint8_t>(blk.spinByte());
                     uint32_t c0 = sLo, c1 = sHi, c2 = static_cast<uint32_t>(b), c3 = 0u;
                     philox10(c0, c1, c2, c3, k0, k1);
                     const uint32_t o[4] = {c0, c1, c2, c3};
@@ -307,16 +360,27 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                         const int v = 8 * b + j;
                         if (tail && v >= n) continue;
                         const uint32_t r = ((j & 1) ? (o[j >> 1] >> 16) : o[j >> 1]);
-                        const uint32_t lb = lutb[r & 15u];
                         const uint32_t fine = (r >> 4) & 0x7Fu;
-                        const int32_t score = scoreOf<kMul24>(beta, blk.field(j), lb, fine);
+                        Score score;
+                        if constexpr (kWide)
+                            score = scoreWide(beta, blk.field(j), lutw[r & 15u], fine);
+                        else
+                            score = scoreOf<kMul24>(beta, blk.field(j), lutb[r & 15u], fine);
                         if constexpr (kLane) {
                             // slot j keeps the block index; the vertex is 8*b + j
                             if (score > accS[j]) { accS[j] = score; accV[j] = b; }
                         } else {
                             const int g = pow2 ? (v & (P - 1)) : (v % P);
-                            atomicMax(reinterpret_cast<unsigned long long*>(&part[g]),
-                                      static_cast<unsigned long long>(packKey(score, v)));
+                            if constexpr (kWide) {
+                                if (!idPass)
+                                    atomicMax(reinterpret_cast<unsigned long long*>(&part[g]),
+                                              static_cast<unsigned long long>(biased64(score)));
+                                else if (biased64(score) == part[g])
+                                    atomicMax(&partId[g], ~static_cast<uint32_t>(v));
+                            } else {
+                                atomicMax(reinterpret_cast<unsigned long long*>(&part[g]),
+                                          static_cast<unsigned long long>(packKey(score, v)));
+                            }
                         }
                     }
                 };
@@ -331,15 +395,14 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                         for (int m = Q; m < 64; m <<= 1) {
 #pragma unroll
                             for (int j = 0; j < 8; ++j) {
-                                const int32_t s2 = __shfl_xor(accS[j], m, 64);
-                                const int32_t v2 = __shfl_xor(accV[j], m, 64);
+                                const Score s2 = shflXor(accS[j], m);
+                                const int32_t v2 = shflXor(accV[j], m);
                                 keepBetter(accS[j], accV[j], s2, v2);
                             }
                         }
                         if (lane < Q) {
 #pragma unroll
-                            for (int j = 0; j < 8; ++j)
-                                part[wave * pstride + 8 * lane + j] = packKey(accS[j], accV[j]);
+                            for (int j = 0; j < 8; ++j) putPartial(wave * pstride + 8 * lane + j, accS[j], accV[j]);
                         }
                     } else {
                         // P in {1, 2, 4}: slot j feeds group j mod P.
@@ -356,25 +419,35 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                         for (int m = 1; m < 64; m <<= 1) {
 #pragma unroll
                             for (int j = 0; j < 4; ++j) {
-                                const int32_t s2 = __shfl_xor(accS[j], m, 64);
-                                const int32_t v2 = __shfl_xor(accV[j], m, 64);
+                                const Score s2 = shflXor(accS[j], m);
+                                const int32_t v2 = shflXor(accV[j], m);
                                 keepBetter(accS[j], accV[j], s2, v2);
                             }
                         }
                         if (lane == 0) {
 #pragma unroll
                             for (int j = 0; j < 4; ++j)
-                                if (j < P) part[wave * pstride + j] = packKey(accS[j], accV[j]);
+                                if (j < P) putPartial(wave * pstride + j, accS[j], accV[j]);
                         }
                     }
                 }
             };
-            if (laneReduce) select(std::true_type{}); else select(std::false_type{});
+            if (laneReduce) {
+                select(std::true_type{});
+            } else {
+                select(std::false_type{});
+                if constexpr (kWide) {
+                    __syncthreads();   // every group's max score is in its slot
+                    idPass = true;
+                    select(std::false_type{});
+                }
+            }
             __syncthreads();   // B1: all partials / slots written
 
             // ---------------- RESOLVE: winner of each group ----------------
             for (int g = tid; g < P; g += NT) {
                 uint64_t key = 0ull;
+                uint32_t kid = 0u;   // ~id of the winner (int64 scores)
                 // Every entry read here is cleared again, so the region is all
                 // zero between sweeps whichever reduction the next phase uses.
                 if (laneReduce) {
@@ -382,15 +455,25 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                     for (int w = 0; w < NW; ++w) {
                         const uint64_t k2 = part[w * pstride + g];
                         part[w * pstride + g] = 0ull;
-                        key = k2 > key ? k2 : key;
+                        if constexpr (kWide) {
+                            const uint32_t i2 = partId[w * pstride + g];
+                            partId[w * pstride + g] = 0u;
+                            if (k2 > key || (k2 == key && i2 > kid)) { key = k2; kid = i2; }
+                        } else {
+                            key = k2 > key ? k2 : key;
+                        }
                     }
                 } else {
                     key = part[g];
                     part[g] = 0ull;
+                    if constexpr (kWide) {
+                        kid = partId[g];
+                        partId[g] = 0u;
+                    }
                 }
                 int32_t pick = -1, sp = 0, dl = 0;
                 if (key != 0ull) {
-                    pick = static_cast<int32_t>(~static_cast<uint32_t>(key));
+                    pick = static_cast<int32_t>(~(kWide ? kid : static_cast<uint32_t>(key)));
                     const uint32_t w = ldsWord<kBits>(wl, pick);
                     sp = (w & F::kSpin) ? 1 : -1;
                     dl = static_cast<int32_t>(w & F::kField) - F::kBias;
@@ -408,9 +491,7 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                 runStop = static_cast<uint64_t>(uni(betaEnd[run]));
                 betaNext = uni(betaVal[run]);
             }
-            if (tid < 16)
-                lutb[tid] = static_cast<uint32_t>(kGumbelCoarse[tid]) -
-                            static_cast<uint32_t>(F::kBias) * static_cast<uint32_t>(betaNext);
+            setLut(betaNext);
 
             // ---------------- FLIP ----------------
             int64_t dsum = 0;
@@ -587,30 +668,42 @@ hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s) {
     return hipGetLastError();
 }
 
-// The compiled variants: word width x 24-bit multiply x threads per chain.
-template <int kBits, bool kMul24>
+// The compiled variants: word width x score mode x threads per chain.
+template <int kBits, int kMode>
 static const void* kernelFor(int threads) {
     switch (threads) {
-        case 128: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMul24, 128>);
-        case 256: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMul24, 256>);
-        case 512: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMul24, 512>);
-        case 1024: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMul24, 1024>);
+        case 128: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 128>);
+        case 256: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 256>);
+        case 512: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 512>);
+        case 1024: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 1024>);
         default: return nullptr;
     }
 }
 
-static const void* kernelFor(int bits, bool mul24, int threads) {
+template <int kBits>
+static const void* kernelForMode(int mode, int threads) {
+    switch (mode) {
+        case dev::kScore32: return kernelFor<kBits, dev::kScore32>(threads);
+        case dev::kScore24:   // 24-bit fields only (8/16-bit words)
+            if constexpr (kBits <= 16) return kernelFor<kBits, dev::kScore24>(threads);
+            return nullptr;
+        case dev::kScore64: return kernelFor<kBits, dev::kScore64>(threads);
+        default: return nullptr;
+    }
+}
+
+static const void* kernelFor(int bits, int mode, int threads) {
     switch (bits) {
-        case 8:  return mul24 ? kernelFor<8, true>(threads) : kernelFor<8, false>(threads);
-        case 16: return mul24 ? kernelFor<16, true>(threads) : kernelFor<16, false>(threads);
-        case 32: return kernelFor<32, false>(threads);
+        case 8:  return kernelForMode<8>(mode, threads);
+        case 16: return kernelForMode<16>(mode, threads);
+        case 32: return kernelForMode<32>(mode, threads);
         default: return nullptr;
     }
 }
 
-hipError_t anneal(int bits, bool mul24, int threads, const ChainItem* items, int nItems, uint32_t* sched,
+hipError_t anneal(int bits, int mode, int threads, const ChainItem* items, int nItems, uint32_t* sched,
                   const LdsPlan& plan, int grid, uint64_t sliceSweeps, hipStream_t s) {
-    const void* kern = kernelFor(bits, mul24, threads);
+    const void* kern = kernelFor(bits, mode, threads);
     if (!kern) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        static_cast<int>(plan.total));
@@ -620,8 +713,8 @@ hipError_t anneal(int bits, bool mul24, int threads, const ChainItem* items, int
     return hipLaunchKernel(kern, dim3(grid), dim3(threads), args, plan.total, s);
 }
 
-int maxActiveBlocks(int bits, bool mul24, int threads, size_t lds) {
-    const void* kern = kernelFor(bits, mul24, threads);
+int maxActiveBlocks(int bits, int mode, int threads, size_t lds) {
+    const void* kern = kernelFor(bits, mode, threads);
     if (!kern) return 0;
     if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) !=
         hipSuccess)

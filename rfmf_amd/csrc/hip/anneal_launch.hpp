@@ -16,8 +16,17 @@ namespace dev {
 constexpr int kThreadChoices[4] = {128, 256, 512, 1024};
 constexpr int kThreads = 256;
 inline bool validThreads(int nt) { return nt == 128 || nt == 256 || nt == 512 || nt == 1024; }
-// Static LDS of the slice kernel (the 16-entry biased Gumbel LUT).
-constexpr uint32_t kStaticLds = 64;
+// Static LDS of the slice kernel (the 16-entry biased Gumbel LUT, 32-bit
+// for the int32 score modes, 64-bit for the int64 one).
+constexpr uint32_t kStaticLds = 128;
+
+// How a vertex's score = Gumbel + beta * spin*S is computed:
+//   kScore32: int32 arithmetic mod 2^32 (exact whenever the int32 gate holds);
+//   kScore24: the same with one v_mad_i32_i24 (|beta| < 2^23, 8/16-bit words);
+//   kScore64: exact int64 scores, for schedules the int32 gate refuses
+//             (the reference's two-pass path, src/anneal_kernels.cu:350-378).
+enum ScoreMode : int { kScore32 = 0, kScore24 = 1, kScore64 = 2 };
+constexpr int kScoreModes = 3;
 
 // Byte offsets of the LDS regions of one workgroup (host-computed from the
 // largest graph and the largest P of the launch).
@@ -25,6 +34,7 @@ struct LdsPlan {
     uint32_t wordsOff;    // packed words of the resident chain
     uint32_t bestOff;     // best-spin bitmask
     uint32_t partOff;     // per-wave group partials / atomic slots (u64)
+    uint32_t partIdOff;   // int64 scores only: ~id beside each partial (u32)
     uint32_t pickOff;     // int32 winner id per group
     uint32_t spinOff;     // int32 winner pre-flip spin per group
     uint32_t deltaOff;    // int32 winner pre-flip delta per group
@@ -49,7 +59,7 @@ struct InitItem {
 
 inline uint32_t alignUp(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
 
-inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads) {
+inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads, int mode) {
     LdsPlan p{};
     const int waves = threads / 64;
     const int laneP = maxP < kMaxLaneReduceP ? maxP : kMaxLaneReduceP;
@@ -59,6 +69,8 @@ inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads) {
     p.wordsOff = off; off += alignUp(static_cast<uint32_t>(wordBytes(maxN, bits)), 16);
     p.bestOff = off;  off += alignUp(static_cast<uint32_t>(bestBytes(maxN)), 16);
     p.partOff = off;  off += alignUp(static_cast<uint32_t>(slots) * 8u, 16);
+    p.partIdOff = off;
+    if (mode == kScore64) off += alignUp(static_cast<uint32_t>(slots) * 4u, 16);
     p.pickOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
     p.spinOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
     p.deltaOff = off; off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
@@ -77,12 +89,12 @@ namespace launch {
 // Persistent slice kernel over `nItems` chain items, each advanced by at most
 // `sliceSweeps` sweeps, `threads` threads per workgroup; `sched` is a 2-word
 // device counter pair, zero before the launch and left zero after it.
-hipError_t anneal(int bits, bool mul24, int threads, const dev::ChainItem* items, int nItems,
+hipError_t anneal(int bits, int mode, int threads, const dev::ChainItem* items, int nItems,
                   uint32_t* sched, const dev::LdsPlan& plan, int grid, uint64_t sliceSweeps,
                   hipStream_t s);
 
 // Resident workgroups per CU for a variant at a given dynamic LDS size.
-int maxActiveBlocks(int bits, bool mul24, int threads, size_t lds);
+int maxActiveBlocks(int bits, int mode, int threads, size_t lds);
 
 // Device-to-device copy of `bytes` (a multiple of 16) with a plain kernel.
 hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s);

@@ -293,12 +293,10 @@ rfmf_status rfmf_anneal(const rfmf_csr* g, int64_t max_weighted_deg, const int8_
         const int64_t total = annealTotalSteps(it);
         const std::vector<double> t0(t0s, t0s + chains);
         // Tables first: an over-cap request fails before any device work.
+        // The int32 gate is judged at the tables' largest beta (the one of
+        // min t0, src/anneal_kernels.cu:637-639) with the caller's B; when it
+        // fails the chains score in exact int64 instead, with the same result.
         const BetaRunTables beta = buildBetaRunTables(t0, total);
-        const double minT0 = *std::min_element(t0.begin(), t0.end());
-        if (!annealScoreSafeInt32(total, minT0, std::max<int64_t>(max_weighted_deg, 0)))
-            throw Error(Status::Unsupported,
-                        "the int32 score gate fails at min t0; the int64 two-pass score path is not "
-                        "implemented");
         annealOneShot(g, max_weighted_deg, initial_spins, chains, beta.runEnd.data(), beta.runVal.data(),
                       beta.chainRunOff.data(), static_cast<int64_t>(beta.runEnd.size()), -1, phases, iters,
                       n_phases, solver_seeds, cfg, end_spins, best_spins, kernel_ms, kernel_best_vals,

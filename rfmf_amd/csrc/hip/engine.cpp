@@ -197,29 +197,30 @@ const DevGraph* Context::graph(int32_t id) const {
     return it->second.get();
 }
 
-int Context::occupancy(int bits, bool mul24, int threads, size_t lds) {
-    const uint64_t key = (static_cast<uint64_t>(bits) << 48) | (static_cast<uint64_t>(mul24) << 47) |
+int Context::occupancy(int bits, int mode, int threads, size_t lds) {
+    const uint64_t key = (static_cast<uint64_t>(bits) << 48) | (static_cast<uint64_t>(mode) << 44) |
                          (static_cast<uint64_t>(threads) << 32) | lds;
     const auto it = occCache_.find(key);
     if (it != occCache_.end()) return it->second;
-    const int occ = launch::maxActiveBlocks(bits, mul24, threads, lds);
+    const int occ = launch::maxActiveBlocks(bits, mode, threads, lds);
     occCache_[key] = occ;
     return occ;
 }
 
-uint64_t Context::tuneKey(int bits, int64_t maxN, int64_t chains, int maxP) {
+uint64_t Context::tuneKey(int bits, int mode, int64_t maxN, int64_t chains, int maxP) {
     auto lg = [](int64_t x) { int l = 0; while ((int64_t{1} << (l + 1)) <= x) ++l; return static_cast<uint64_t>(l); };
-    return (static_cast<uint64_t>(bits) << 40) | (lg(std::max<int64_t>(maxN, 1)) << 32) |
-           (lg(std::max<int64_t>(chains, 1)) << 24) | static_cast<uint64_t>(maxP);
+    return (static_cast<uint64_t>(bits) << 42) | (static_cast<uint64_t>(mode) << 40) |
+           (lg(std::max<int64_t>(maxN, 1)) << 32) | (lg(std::max<int64_t>(chains, 1)) << 24) |
+           static_cast<uint64_t>(maxP);
 }
 
-bool Context::fitsLds(int64_t maxN, int bits, int maxP, int threads) const {
-    const dev::LdsPlan p = dev::makeLdsPlan(maxN, bits, maxP, threads);
+bool Context::fitsLds(int64_t maxN, int bits, int mode, int maxP, int threads) const {
+    const dev::LdsPlan p = dev::makeLdsPlan(maxN, bits, maxP, threads, mode);
     return static_cast<size_t>(p.total) + dev::kStaticLds <= props_.sharedMemPerBlock;
 }
 
-int Context::chooseThreads(int bits, int64_t maxN, int64_t chains, int maxP) const {
-    const auto it = tuned_.find(tuneKey(bits, maxN, chains, maxP));
+int Context::chooseThreads(int bits, int mode, int64_t maxN, int64_t chains, int maxP) const {
+    const auto it = tuned_.find(tuneKey(bits, mode, maxN, chains, maxP));
     if (it != tuned_.end()) return it->second;
     // Heuristic: with several chains per CU the 4-wave tile keeps every SIMD
     // busy across chains; with fewer chains than CUs, more waves per chain
@@ -229,7 +230,7 @@ int Context::chooseThreads(int bits, int64_t maxN, int64_t chains, int maxP) con
     if (perCu >= 2.0) t = maxN <= 2048 ? 128 : 256;
     else if (perCu >= 1.0) t = 512;
     else t = maxN >= 4096 ? 1024 : 512;
-    while (t > 128 && !fitsLds(maxN, bits, maxP, t)) t /= 2;
+    while (t > 128 && !fitsLds(maxN, bits, mode, maxP, t)) t /= 2;
     return t;
 }
 
@@ -308,19 +309,22 @@ HostJob Context::prepareJob(const JobSpec& spec, const HostGraph& g) const {
     for (int64_t c = 0; c < C; ++c) hj.tableLen[c] = static_cast<int32_t>(tableLen(hj.tableStart[c]));
     if (spec.gateBetaMax >= 0) betaMax = spec.gateBetaMax;
 
-    const int64_t gateB = std::max<int64_t>(spec.gateB, g.B);
-    if (!scoreSafeInt32ForBeta(betaMax, gateB))
-        throw Error(Status::Unsupported,
-                    "the int32 score gate fails (beta_max*B + |gumbel|max >= INT32_MAX); the int64 "
-                    "two-pass score path is not implemented");
     hj.bits = wordBitsFor(g.B);
     if (hj.bits == 0)
         throw Error(Status::Unsupported, "max weighted degree >= 2^30 is not supported");
-    // 24-bit multiplies need |beta| < 2^23 and a field below 2^23 (8/16-bit words).
-    hj.mul24 = betaMax < (int64_t{1} << 23) && hj.bits <= 16;
+    // Score arithmetic: int32 when the gate holds (src/host_common.cpp:296-312,
+    // verdict at the largest beta), with 24-bit multiplies when |beta| < 2^23
+    // and the field is below 2^23 (8/16-bit words); exact int64 otherwise.
+    const int64_t gateB = std::max<int64_t>(spec.gateB, g.B);
+    if (!scoreSafeInt32ForBeta(betaMax, gateB))
+        hj.mode = dev::kScore64;
+    else if (betaMax < (int64_t{1} << 23) && hj.bits <= 16)
+        hj.mode = dev::kScore24;
+    else
+        hj.mode = dev::kScore32;
     // The smallest tile (128 threads) has the smallest plan; launches pick a
     // tile that fits.
-    const dev::LdsPlan plan = dev::makeLdsPlan(n, hj.bits, maxP, 128);
+    const dev::LdsPlan plan = dev::makeLdsPlan(n, hj.bits, maxP, 128, hj.mode);
     if (static_cast<size_t>(plan.total) + dev::kStaticLds > props_.sharedMemPerBlock)
         throw Error(Status::Unsupported, "chain state of n=" + std::to_string(n) + " (" +
                                              std::to_string(plan.total) +
@@ -351,7 +355,7 @@ std::unique_ptr<ResidentJob> Context::placeJob(const HostJob& hj, const DevGraph
     job->totalSteps = hj.totalSteps;
     job->maxP = hj.maxP;
     job->bits = hj.bits;
-    job->mul24 = hj.mul24;
+    job->mode = hj.mode;
 
     hipCheck(hipSetDevice(device_), "hipSetDevice");
     hipMemPool_t pool = streamOrdered ? this->pool() : nullptr;
@@ -442,12 +446,13 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
                                                       int threads) {
     // Group unfinished chains by kernel variant; items sorted by descending n
     // so the persistent workgroups take the longest chains first.
-    std::vector<LaunchGroup> groups(6);
-    std::vector<int64_t> maxN(6, 0);
-    std::vector<int> maxP(6, 1);
-    auto gidx = [](int bits, bool mul24) { return (bits == 8 ? 0 : bits == 16 ? 1 : 2) * 2 + (mul24 ? 1 : 0); };
+    constexpr int kGroups = 3 * dev::kScoreModes;
+    std::vector<LaunchGroup> groups(kGroups);
+    std::vector<int64_t> maxN(kGroups, 0);
+    std::vector<int> maxP(kGroups, 1);
+    auto gidx = [](int bits, int mode) { return (bits == 8 ? 0 : bits == 16 ? 1 : 2) * dev::kScoreModes + mode; };
     for (ResidentJob* j : jobs) {
-        const int gi = gidx(j->bits, j->mul24);
+        const int gi = gidx(j->bits, j->mode);
         for (size_t c = 0; c < j->items.size(); ++c) {
             if (j->stepsDone[c] >= j->totalSteps) continue;
             groups[gi].items.push_back(j->items[c]);
@@ -456,22 +461,23 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
         }
     }
     std::vector<LaunchGroup> out;
-    for (int gi = 0; gi < 6; ++gi) {
+    for (int gi = 0; gi < kGroups; ++gi) {
         LaunchGroup& g = groups[gi];
         if (g.items.empty()) continue;
         std::stable_sort(g.items.begin(), g.items.end(),
                          [](const dev::ChainItem& a, const dev::ChainItem& b) { return a.n > b.n; });
-        g.bits = (gi / 2 == 0) ? 8 : (gi / 2 == 1) ? 16 : 32;
-        g.mul24 = (gi % 2) == 1;
+        g.bits = (gi / dev::kScoreModes == 0) ? 8 : (gi / dev::kScoreModes == 1) ? 16 : 32;
+        g.mode = gi % dev::kScoreModes;
         g.threads = threads > 0 ? threads
-                                : chooseThreads(g.bits, maxN[gi], static_cast<int64_t>(g.items.size()), maxP[gi]);
+                                : chooseThreads(g.bits, g.mode, maxN[gi], static_cast<int64_t>(g.items.size()),
+                                                maxP[gi]);
         if (!dev::validThreads(g.threads))
             throw Error(Status::InvalidArg, "threads per chain must be 128, 256, 512 or 1024");
-        g.plan = dev::makeLdsPlan(maxN[gi], g.bits, maxP[gi], g.threads);
+        g.plan = dev::makeLdsPlan(maxN[gi], g.bits, maxP[gi], g.threads, g.mode);
         if (static_cast<size_t>(g.plan.total) + dev::kStaticLds > props_.sharedMemPerBlock)
             throw Error(Status::Unsupported, "combined launch needs " + std::to_string(g.plan.total) +
                                                  " B of LDS per workgroup");
-        const int occ = std::max(1, occupancy(g.bits, g.mul24, g.threads, g.plan.total));
+        const int occ = std::max(1, occupancy(g.bits, g.mode, g.threads, g.plan.total));
         int grid = occ * props_.multiProcessorCount;
         if (gridCap > 0) grid = std::min(grid, gridCap);
         g.grid = static_cast<int>(std::min<int64_t>(grid, static_cast<int64_t>(g.items.size())));
@@ -514,7 +520,7 @@ Context::SliceHandle Context::launchSlice(const std::vector<ResidentJob*>& jobs,
         at = 0;
         int ctr = 0;
         for (LaunchGroup& g : groups) {
-            hipCheck(launch::anneal(g.bits, g.mul24, g.threads, slot.dev.as<dev::ChainItem>() + at,
+            hipCheck(launch::anneal(g.bits, g.mode, g.threads, slot.dev.as<dev::ChainItem>() + at,
                                     static_cast<int>(g.items.size()), sched_.as<uint32_t>() + 2 * ctr, g.plan,
                                     g.grid, slice, stream_), "annealSlice launch");
             at += g.items.size();
@@ -547,14 +553,14 @@ std::unique_ptr<Context::GraphedSlices> Context::captureSlices(const std::vector
         g->itemBufs.emplace_back(lg.items.size() * sizeof(dev::ChainItem));
         hipCheck(hipMemcpy(g->itemBufs.back().get(), lg.items.data(), g->itemBufs.back().bytes(),
                            hipMemcpyHostToDevice), "upload items");
-        (void)launch::maxActiveBlocks(lg.bits, lg.mul24, lg.threads, lg.plan.total);   // LDS attribute, outside the capture
+        (void)launch::maxActiveBlocks(lg.bits, lg.mode, lg.threads, lg.plan.total);   // LDS attribute, outside the capture
     }
     hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     int slot = 0;
     hipError_t err = hipSuccess;
     for (size_t i = 0; i < groups.size() && err == hipSuccess; ++i, ++slot)
-        err = launch::anneal(groups[i].bits, groups[i].mul24, groups[i].threads, g->itemBufs[i].as<dev::ChainItem>(),
+        err = launch::anneal(groups[i].bits, groups[i].mode, groups[i].threads, g->itemBufs[i].as<dev::ChainItem>(),
                              static_cast<int>(groups[i].items.size()), sched_.as<uint32_t>() + 2 * slot,
                              groups[i].plan, groups[i].grid, sliceSweeps, stream_);
     const hipError_t endErr = hipStreamEndCapture(stream_, &g->graph);
@@ -596,21 +602,22 @@ float Context::waitSlice(SliceHandle& h) {
 }
 
 int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* trialMs) {
-    // Shape of the launch: widest word, largest n and P, unfinished chains.
-    int bits = 8, maxP = 1;
+    // Shape of the launch: widest word and score, largest n and P, unfinished chains.
+    int bits = 8, mode = 0, maxP = 1;
     int64_t maxN = 0, chains = 0;
     uint64_t minLeft = ~uint64_t{0};
     for (const ResidentJob* j : jobs)
         for (size_t c = 0; c < j->items.size(); ++c) {
             if (j->stepsDone[c] >= j->totalSteps) continue;
             bits = std::max(bits, j->bits);
+            mode = std::max(mode, j->mode);
             maxN = std::max<int64_t>(maxN, j->graph->n);
             maxP = std::max(maxP, j->maxP);
             ++chains;
             minLeft = std::min(minLeft, j->totalSteps - j->stepsDone[c]);
         }
     if (chains == 0) return 0;
-    const uint64_t key = tuneKey(bits, maxN, chains, maxP);
+    const uint64_t key = tuneKey(bits, mode, maxN, chains, maxP);
     const auto hit = tuned_.find(key);
     last_.tuned = true;
     if (hit != tuned_.end()) return hit->second;
@@ -622,7 +629,7 @@ int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& 
     float bestPerSweep = 0.0f;
     for (int i = 0; i < 4; ++i) {
         const int t = dev::kThreadChoices[i];
-        if (!fitsLds(maxN, bits, maxP, t)) continue;
+        if (!fitsLds(maxN, bits, mode, maxP, t)) continue;
         LaunchKnobs k = knobs;
         k.threads = t;
         k.sliceSweeps = static_cast<int64_t>(trial);

@@ -122,7 +122,7 @@ struct HostJob {
     int64_t chains = 0;
     int64_t n = 0;
     int bits = 32;
-    bool mul24 = false;
+    int mode = dev::kScore32;   // score arithmetic (dev::ScoreMode)
     int maxP = 1;
     uint64_t totalSteps = 0;
     std::vector<uint64_t> phaseEnd;
@@ -142,7 +142,7 @@ struct ResidentJob {
     const DevGraph* graph = nullptr;
     int64_t chains = 0;
     int bits = 32;
-    bool mul24 = false;
+    int mode = dev::kScore32;   // score arithmetic (dev::ScoreMode)
     int maxP = 1;
     uint64_t totalSteps = 0;   // executed sweeps per chain
     DevBuf words, best, states, runEnd, runVal, phaseEnd, phaseP;
@@ -240,10 +240,10 @@ public:
     void collect(const ResidentJob& job, const JobSpec& spec, hipStream_t stream);
 
     // Resident workgroups per CU of a kernel variant (cached).
-    int occupancy(int bits, bool mul24, int threads, size_t lds);
+    int occupancy(int bits, int mode, int threads, size_t lds);
     // Threads per chain for a launch shape: the autotuner's cached choice,
     // else a heuristic from chains per CU and n.
-    int chooseThreads(int bits, int64_t maxN, int64_t chains, int maxP) const;
+    int chooseThreads(int bits, int mode, int64_t maxN, int64_t chains, int maxP) const;
     const LaunchReport& lastLaunch() const { return last_; }
 
     std::mutex& mutex() { return mu_; }
@@ -252,14 +252,14 @@ private:
     struct LaunchGroup {
         std::vector<dev::ChainItem> items;
         int bits = 8;
-        bool mul24 = false;
+        int mode = dev::kScore32;
         int threads = dev::kThreads;
         dev::LdsPlan plan{};
         int grid = 1;
     };
     std::vector<LaunchGroup> planGroups(const std::vector<ResidentJob*>& jobs, int gridCap, int threads);
-    static uint64_t tuneKey(int bits, int64_t maxN, int64_t chains, int maxP);
-    bool fitsLds(int64_t maxN, int bits, int maxP, int threads) const;
+    static uint64_t tuneKey(int bits, int mode, int64_t maxN, int64_t chains, int maxP);
+    bool fitsLds(int64_t maxN, int bits, int mode, int maxP, int threads) const;
     std::unordered_map<uint64_t, int> tuned_;
     LaunchReport last_;
 

@@ -55,6 +55,8 @@ def _mixed_jobs(graphs):
         ("G70", make_job(graphs["G70"], [6], [8, 2, 1], [50, 50, 50], linear=False, t0=[105.0])),
         ("deg", make_job(graphs["deg"], [7, 8, 9, 10], [3, 64, 1], [40, 40, 40])),
         ("torus", make_job(graphs["torus"], [11, 12], LADDER, [30] * 8)),
+        # beta_max * B + |gumbel| max >= 2^31 - 1: int64 scores, its own launch group
+        ("G11", make_job(graphs["G11"], [15, 16], [8, 4, 2, 1], [60] * 4, linear=False, t0=[0.005, 0.01])),
     ]
 
 
@@ -148,6 +150,8 @@ def test_server_results_equal_one_shot(graphs):
         ("deg", make_job(graphs["deg"], [37], [3, 64, 1], [100, 100, 100])),
         ("G70", make_job(graphs["G70"], [38, 39], [8, 2, 1], [60, 60, 60], linear=False, t0=[90.0, 100.0])),
         ("G11b", make_job(graphs["G11"], [40], [16, 1], [300, 300])),
+        # the int32 gate fails (t0 = 0.005, B = 4): int64 scores
+        ("G11w", make_job(graphs["G11"], [42, 43], [8, 4, 2, 1], [80] * 4, linear=False, t0=[0.005, 0.01])),
     ]
     fps = [R.footprint_bytes(j.csr.n, j.csr.m2, j.chains, j.beta.run_end.size, len(j.phases)) for _, j in reqs]
     budget = sum(sorted(fps)[-2:])   # at most about two requests resident: queueing is exercised
@@ -162,11 +166,11 @@ def test_server_results_equal_one_shot(graphs):
             with pytest.raises(R.RfmfError) as ei:
                 srv.submit(big)
             assert ei.value.status == 10
-            # beta_max * B + |gumbel| max >= 2^31 - 1 (t0 = 0.005, B = 4): the
-            # int32 gate refuses the request at submit, before any device work.
-            gated = make_job(graphs["G11"], [41], [8, 1], [300, 300], linear=False, t0=[0.005])
+            # P above the 4096-group limit: refused at submit, before any
+            # device work.
+            too_many_groups = make_job(graphs["G11"], [41], [5000, 1], [10, 10])
             with pytest.raises(R.RfmfError) as ei:
-                srv.submit(gated)
+                srv.submit(too_many_groups)
             assert ei.value.status == 2
             for rid in ids:
                 srv.wait(rid, 120.0)

@@ -111,3 +111,67 @@ def test_p_zero_phase_skipped(gset):
     csr = gset("G11")
     phases, iters = [8, 0, 4], [100, 50, 100]
     _case("P0-phase", csr, [1, 2], _log_tables([100.0, 100.0], iters), phases, iters)
+
+
+# ---------------------------------------------------------- int64 scores
+# Schedules the int32 gate refuses (beta_max * B + |gumbel| max >= 2^31 - 1)
+# take exact int64 scores; the reference's own isolation test runs one at
+# t0 = 0.01 (tests/test_chain_isolation.sh:131).
+
+def test_int64_scores_small_t0(gset):
+    csr = gset("G11")
+    phases, iters = [8, 4, 2, 1], [100] * 4
+    assert not R.anneal_score_safe_int32(sum(iters), 0.005, R.max_weighted_degree(csr))
+    _case("int64-G11", csr, [1, 2, 3], _log_tables([0.005, 0.01, 0.02], iters), phases, iters)
+
+
+def test_int64_scores_u16_words(gset):
+    csr = gset("G1")
+    phases, iters = [64, 16, 4, 1], [60] * 4
+    assert not R.anneal_score_safe_int32(sum(iters), 0.02, R.max_weighted_degree(csr))
+    _case("int64-G1", csr, [5, 6], _log_tables([0.02, 0.05], iters), phases, iters)
+
+
+@pytest.mark.parametrize("P", [3, 100, 1000])
+def test_int64_scores_general_group_counts(P):
+    # the two-pass argmax over LDS slots (max score, then the smallest id)
+    rng = np.random.default_rng(100 + P)
+    n, m = 613, 2400
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    keep = u != v
+    w = rng.integers(1, 6, m) * np.where(rng.random(m) < 0.5, -1, 1)
+    csr = R.graph_from_edges(u[keep], v[keep], w[keep])
+    phases, iters = [P, 1], [60, 30]
+    assert not R.anneal_score_safe_int32(sum(iters), 0.02, R.max_weighted_degree(csr))
+    _case(f"int64-P={P}", csr, [5, 6, 7], _log_tables([0.02, 0.03, 0.05], iters), phases, iters)
+
+
+def test_int64_scores_u32_words():
+    rng = np.random.default_rng(12)
+    n = 300
+    u = np.arange(n - 1)
+    v = u + 1
+    w = rng.integers(-3000, 3001, n - 1)
+    w[w == 0] = 1
+    u = np.concatenate([u, np.zeros(10, np.int64)])
+    v = np.concatenate([v, np.arange(2, 12)])
+    w = np.concatenate([w, np.full(10, 2000)])
+    csr = R.graph_from_edges(u, v, w)
+    phases, iters = [16, 4, 1], [60, 60, 60]
+    beta = R.build_linear_beta_table(0.0, 1.0, sum(iters), 2)   # beta up to 2^20: 2^20 * B > 2^31
+    assert int(beta.run_val.max()) * R.max_weighted_degree(csr) > 2**31
+    _case("int64-u32-words", csr, [3, 4], beta, phases, iters)
+
+
+def test_int64_scores_equal_int32_scores_when_forced():
+    # B passed to the drop-in call decides the gate (as annealGpu's
+    # maxWeightedDeg does): a B of 2^29 forces int64 scores on a schedule the
+    # int32 path handles; both paths must give identical chains.
+    csr = W.erdos_renyi(800, 0.06, seed=2)
+    phases, iters = [128, 32, 8, 2, 1], [80] * 5
+    beta = R.build_linear_beta_table(0.1, 3.0, sum(iters), 4)
+    narrow = _case("int32", csr, [1, 2, 3, 4], beta, phases, iters)
+    wide = _case("forced-int64", csr, [1, 2, 3, 4], beta, phases, iters, B=2**29)
+    assert np.array_equal(narrow.end_spins, wide.end_spins)
+    assert np.array_equal(narrow.best_spins, wide.best_spins)

@@ -182,9 +182,12 @@ def test_anneal_rejects_bad_inputs_before_device_work():
     with pytest.raises(R.RfmfError) as e:
         R.anneal_gpu(csr, 4, init, [1.0, 2.0], [8, 1], [10**12, 1], seeds)
     assert e.value.status == 7                         # total-sweep cap
+    # A schedule the int32 gate refuses is accepted (int64 scores): on a host
+    # without a GPU the only error left is the missing device.
+    assert not R.anneal_score_safe_int32(100, 0.001, 4)
     with pytest.raises(R.RfmfError) as e:
-        R.anneal_gpu(csr, 4, init, [0.001, 2.0], [8], [10**9], seeds)
-    assert e.value.status in (2, 6)                    # int64 path / run cap: both before any device call
+        R.anneal_gpu(csr, 4, init, [0.001, 2.0], [8], [100], seeds)
+    assert e.value.status == 4
 
 
 def test_anneal_passthrough_without_phases():

@@ -6,8 +6,9 @@
 //           rows sorted by neighbour id (the reference's AnnealCsr order,
 //           src/immutable_graph.hpp:83-101).  Read-only, shared by every chain
 //           of the graph.
-//   chain   words   : packed per-vertex state, one Word per vertex
-//                     (spin bit | S + bias), padded to 16 B.
+//   chain   words   : packed per-vertex state, one word per vertex
+//                     (spin bit | delta + bias, delta = spin*S = the cut gain
+//                     of flipping the vertex), padded to 16 B.
 //           bestbits: 1 bit per vertex (bit v&31 of u32 v>>5), padded to 16 B.
 //           ChainState (32 B): running cut, best cut, executed sweeps,
 //                     improved flag.
@@ -54,11 +55,11 @@ struct alignas(16) ChainItem {
 };
 static_assert(sizeof(ChainItem) == 112, "ChainItem layout");
 
-// Word encodings: spin bit at the top of the word, S biased to stay
-// non-negative so that 32-bit LDS atomic adds of signed deltas never carry or
-// borrow across neighbouring words packed in the same dword (every partial
-// sum of a neighbour update is itself a valid S, so the biased field stays in
-// range at all times).
+// Word encodings: spin bit at the top of the word, delta = spin*S biased to
+// stay non-negative so that 32-bit LDS atomic adds of signed changes never
+// carry or borrow across neighbouring words packed in the same dword (every
+// partial sum of a neighbour update is itself a valid delta, |delta| <= B, so
+// the biased field stays in range at all times).
 template <int kBits> struct WordFmt;
 template <> struct WordFmt<8> {
     static constexpr uint32_t kSpin = 0x80u, kField = 0x7Fu;

@@ -236,30 +236,45 @@ def bench_er800(args) -> int:
 
 
 def bench_sparse(args) -> int:
-    """Sparse random graphs, 4096 chains each in one persistent launch with the
-    autotuned threads-per-chain: n=10000 m=10000 unit weights, and n=7000
-    m=17150 +/-1."""
+    """Sparse random graphs, 4096 chains each in one persistent launch:
+    n=10000 m=10000 unit weights, and n=7000 m=17150 +/-1.  Flips per sweep
+    come from the quality-gated tuner (256 pilot chains per candidate ladder
+    over the full budget; the fastest ladder whose median best cut is within
+    0.1% of the reference's 128..1 ladder), threads per chain from the
+    geometry autotuner; the 128..1 ladder runs beside it for comparison."""
     import rfmf_amd as R
     from rfmf_amd import workloads as W
     sweeps = args.steps if args.steps != 40000 else 100_000
-    iters = [sweeps // 8] * 8
     C = 4096
+    tops = [8, 16, 32, 64, 128, 256, 512]
     rc = 0
     for name, csr in (("sparse_n10000_m10000_unit", W.sparse_random(10000, 10000, False, seed=0)),
                       ("sparse_n7000_m17150_pm1", W.sparse_random(7000, 17150, True, seed=0))):
-        job = _job(R, csr, range(1, C + 1), iters)
         with R.Context(0) as ctx:
             gid = ctx.upload_graph(csr)
-            ms = ctx.batch_run([(gid, job)], autotune=args.threads == 0, threads=args.threads)
-            geo = ctx.last_launch()
-        ok = all(R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c] for c in range(0, C, 97))
-        rc |= 0 if ok else 1
-        rate = C * sum(iters) / (ms / 1e3)
-        log(f"[sparse] {name}: n={csr.n} {ms:.1f} ms {rate:.3e} chain-steps/s {geo}")
+            t0 = time.perf_counter()
+            tune = ctx.tune_flips(gid, sweeps, tops, pilot_chains=256, seed_base=1_000_000, reference_top_p=128)
+            tune_s = time.perf_counter() - t0
+            out = {}
+            for label, top in (("tuned", tune["chosen_top_p"]), ("ladder128", 128)):
+                phases, iters = R.flips_ladder(top, sweeps)
+                job = R.Job(csr, np.stack([R.generate_random_spins(s, csr.n) for s in range(1, C + 1)]),
+                            [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in range(1, C + 1)],
+                            R.build_linear_beta_table(0.1, 3.0, sweeps, C), phases, iters)
+                ms = ctx.batch_run([(gid, job)], autotune=args.threads == 0, threads=args.threads)
+                geo = ctx.last_launch()
+                ok = all(R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c] for c in range(0, C, 97))
+                rc |= 0 if ok else 1
+                out[label] = {"top_p": top, "value": C * sweeps / (ms / 1e3), "kernel_ms": ms, "geometry": geo,
+                              "best_cut_median": float(np.median(job.best_vals)),
+                              "best_cut_max": int(job.best_vals.max()), "cuts_recomputed": bool(ok)}
+        log(f"[sparse] {name}: n={csr.n} tuner {tune_s:.1f} s -> top P {tune['chosen_top_p']} "
+            f"(pilot medians {tune['median_best']}); tuned {out['tuned']['value']:.3e} chain-steps/s median "
+            f"{out['tuned']['best_cut_median']}, ladder128 {out['ladder128']['value']:.3e} median "
+            f"{out['ladder128']['best_cut_median']}")
         _emit({"metric": "chain-steps/sec", "config": name, "n": csr.n, "m": csr.m2 // 2, "chains": C,
-               "sweeps": sum(iters), "value": rate, "kernel_ms": ms, "geometry": geo,
-               "best_cut_median": float(np.median(job.best_vals)), "best_cut_max": int(job.best_vals.max()),
-               "cuts_recomputed": bool(ok)})
+               "sweeps": sweeps, "value": out["tuned"]["value"], "flips_tuner": tune, "tuner_seconds": tune_s,
+               "runs": out})
     return rc
 
 

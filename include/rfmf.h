@@ -249,6 +249,46 @@ int64_t rfmf_footprint_bytes(int64_t n, int64_t m2, int64_t chains, int64_t beta
                              int64_t n_phases);
 
 /* ---------------------------------------------------------------------- */
+/* Flips per sweep, picked by measurement.  No reference counterpart: the  */
+/* reference takes the group-count ladder as an input (main.cu --phases).  */
+/* ---------------------------------------------------------------------- */
+
+#define RFMF_MAX_FLIPS_CANDIDATES 16
+
+/* The ladder of a top group count: phases P, P/2, ..., 1 (P a power of two)
+ * with `sweeps` split into equal phases (the first sweeps % len phases get one
+ * more).  Call with phases/iters NULL for the length only. */
+rfmf_status rfmf_flips_ladder(uint64_t top_p, int64_t sweeps, uint64_t* phases, uint64_t* iters,
+                              int64_t cap, int64_t* n_phases);
+
+typedef struct rfmf_flips_tune {
+    int64_t sweeps;            /* per-chain sweep budget of the production run */
+    int64_t pilot_chains;      /* chains per candidate; 0: 256 */
+    uint64_t seed_base;        /* pilot chain c has raw seed seed_base + 1 + c */
+    int32_t beta0_q20;         /* linear schedule beta0 -> beta1, Q20 */
+    int32_t beta1_q20;
+    const uint64_t* top_p;     /* candidate top group counts (powers of two) */
+    int32_t n_candidates;      /* 1 .. RFMF_MAX_FLIPS_CANDIDATES */
+    int32_t reference;         /* index of the candidate whose median sets the quality floor; -1: the best */
+    double quality_tol;        /* relative tolerance on the median best cut, e.g. 0.001 */
+} rfmf_flips_tune;
+
+typedef struct rfmf_flips_report {
+    int32_t chosen;            /* index into top_p */
+    int32_t n_candidates;
+    double median_best[RFMF_MAX_FLIPS_CANDIDATES];        /* median incremental best cut of the pilots */
+    double chain_steps_per_s[RFMF_MAX_FLIPS_CANDIDATES];  /* pilot throughput */
+} rfmf_flips_report;
+
+/* Anneals `pilot_chains` chains of every candidate ladder over the full
+ * budget (initial spins and solver seeds by the production rules, autotuned
+ * geometry), then chooses: among candidates whose median best cut is at least
+ * (1 - quality_tol) x the floor median (the `reference` candidate's, or the
+ * best candidate's), the one with the highest chain-steps/s. */
+rfmf_status rfmf_tune_flips(rfmf_ctx* ctx, int32_t graph_id, const rfmf_flips_tune* spec,
+                            rfmf_flips_report* report);
+
+/* ---------------------------------------------------------------------- */
 /* Serving: continuous batching of MAX-CUT solve requests.                 */
 /* ---------------------------------------------------------------------- */
 

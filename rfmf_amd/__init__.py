@@ -25,7 +25,7 @@ __all__ = [
     "Server", "RfmfError", "read_instance_from_file", "graph_from_edges", "cut_value_csr",
     "max_weighted_degree", "anneal_score_safe_int32", "beta_fixed_for_step", "anneal_total_steps",
     "generate_random_spins", "build_beta_run_tables", "build_linear_beta_table", "anneal_gpu",
-    "anneal_gpu_tables", "footprint_bytes", "K_SOLVER_SEED_OFFSET", "Q20",
+    "anneal_gpu_tables", "footprint_bytes", "flips_ladder", "K_SOLVER_SEED_OFFSET", "Q20",
 ]
 
 
@@ -264,6 +264,18 @@ def footprint_bytes(n: int, m2: int, chains: int, beta_runs: int, n_phases: int)
     return int(lib().rfmf_footprint_bytes(n, m2, chains, beta_runs, n_phases))
 
 
+def flips_ladder(top_p: int, sweeps: int) -> tuple[list[int], list[int]]:
+    """The group-count ladder top_p, top_p/2, ..., 1 with `sweeps` split into
+    equal phases (rfmf_flips_ladder)."""
+    n = C.c_int64()
+    check(lib().rfmf_flips_ladder(int(top_p), int(sweeps), None, None, 0, C.byref(n)))
+    ph = np.empty(n.value, dtype=np.uint64)
+    it = np.empty(n.value, dtype=np.uint64)
+    check(lib().rfmf_flips_ladder(int(top_p), int(sweeps), _ptr(ph, C.c_uint64), _ptr(it, C.c_uint64), n.value,
+                                  C.byref(n)))
+    return [int(x) for x in ph], [int(x) for x in it]
+
+
 class Job:
     """Chains of one graph with their inputs and output buffers (rfmf_job)."""
 
@@ -367,6 +379,25 @@ class Context:
         check(lib().rfmf_ctx_last_launch(self._h, C.byref(t), C.byref(g), C.byref(b), C.byref(o), trial))
         return {"threads_per_chain": t.value, "grid": g.value, "word_bits": b.value, "wg_per_cu": o.value,
                 "trial_ms_per_sweep": {n: trial[i] for i, n in enumerate((128, 256, 512, 1024)) if trial[i] > 0}}
+
+    def tune_flips(self, graph_id: int, sweeps: int, top_ps, beta=(0.1, 3.0), pilot_chains: int = 0,
+                   seed_base: int = 0, quality_tol: float = 0.001, reference_top_p: int | None = None) -> dict:
+        """Quality-gated choice of flips per sweep (rfmf_tune_flips): every
+        candidate ladder top_p..1 anneals the same pilot chains over the full
+        budget; among candidates whose median best cut is within quality_tol
+        of the floor median (the reference_top_p ladder's, else the best), the
+        fastest is chosen."""
+        tops = np.ascontiguousarray(top_ps, dtype=np.uint64)
+        ref = -1 if reference_top_p is None else [int(t) for t in tops].index(int(reference_top_p))
+        spec = _capi.FlipsTune(int(sweeps), int(pilot_chains), int(seed_base), int(round(beta[0] * Q20)),
+                               int(round(beta[1] * Q20)), _ptr(tops, C.c_uint64), tops.size, ref,
+                               float(quality_tol))
+        rep = _capi.FlipsReport()
+        check(lib().rfmf_tune_flips(self._h, int(graph_id), C.byref(spec), C.byref(rep)))
+        k = rep.chosen
+        return {"chosen_top_p": int(tops[k]), "chosen": k,
+                "median_best": {int(t): rep.median_best[i] for i, t in enumerate(tops)},
+                "chain_steps_per_s": {int(t): rep.chain_steps_per_s[i] for i, t in enumerate(tops)}}
 
     def batch_run_windows(self, jobs: list[tuple[int, Job]], bounds: list[int]) -> list[float]:
         """Runs jobs with one timed launch per window [bounds[i-1], bounds[i])

@@ -49,6 +49,21 @@ class Tuning(C.Structure):
                 ("threads_per_chain", C.c_int32), ("autotune", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
+MAX_FLIPS_CANDIDATES = 16
+
+
+class FlipsTune(C.Structure):
+    _fields_ = [("sweeps", C.c_int64), ("pilot_chains", C.c_int64), ("seed_base", C.c_uint64),
+                ("beta0_q20", C.c_int32), ("beta1_q20", C.c_int32), ("top_p", u64p),
+                ("n_candidates", C.c_int32), ("reference", C.c_int32), ("quality_tol", C.c_double)]
+
+
+class FlipsReport(C.Structure):
+    _fields_ = [("chosen", C.c_int32), ("n_candidates", C.c_int32),
+                ("median_best", C.c_double * MAX_FLIPS_CANDIDATES),
+                ("chain_steps_per_s", C.c_double * MAX_FLIPS_CANDIDATES)]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
                 ("active_chains", C.c_int64), ("resident_bytes", C.c_int64), ("slices", C.c_int64),
@@ -91,6 +106,8 @@ SIGNATURES = {
     "rfmf_run_collect": (C.c_int, [C.c_void_p]),
     "rfmf_run_destroy": (None, [C.c_void_p]),
     "rfmf_footprint_bytes": (C.c_int64, [C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int64]),
+    "rfmf_flips_ladder": (C.c_int, [C.c_uint64, C.c_int64, u64p, u64p, C.c_int64, i64p]),
+    "rfmf_tune_flips": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(FlipsTune), C.POINTER(FlipsReport)]),
     "rfmf_server_create": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_void_p)]),
     "rfmf_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Csr), C.POINTER(Job), i64p]),
     "rfmf_server_poll": (C.c_int, [C.c_void_p, C.c_int64, i32p, f64p, f64p, f64p]),

@@ -18,29 +18,11 @@
 using rfmf::Error;
 using rfmf::Status;
 
+using rfmf::guarded;
+
 namespace {
 
 thread_local std::string g_lastError;
-
-rfmf_status fail(Status s, const std::string& msg) {
-    g_lastError = msg;
-    return static_cast<rfmf_status>(s);
-}
-
-template <class F>
-rfmf_status guarded(F&& f) {
-    try {
-        g_lastError.clear();
-        f();
-        return RFMF_OK;
-    } catch (const Error& e) {
-        return fail(e.status(), e.what());
-    } catch (const std::bad_alloc&) {
-        return fail(Status::OutOfMemory, "host allocation failed");
-    } catch (const std::exception& e) {
-        return fail(Status::InvalidArg, e.what());
-    }
-}
 
 void requireCsr(const rfmf_csr* g) {
     if (!g || !g->offsets || g->n < 0 || g->m2 < 0 || (g->m2 > 0 && (!g->edge_to || !g->edge_weight)))

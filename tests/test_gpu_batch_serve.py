@@ -180,3 +180,29 @@ def test_server_results_equal_one_shot(graphs):
     assert st["completed"] == len(reqs) and st["rejected"] == 2
     for (k, j) in reqs:
         same(j, one_shot(j), f"served {k}")
+
+
+def test_flips_tuner_gate_and_pilots_are_ordinary_runs(gset):
+    csr = gset("G11")
+    sweeps, tops, pilots = 1600, [4, 16, 64], 24
+    print(f"[tune-flips] G11 sweeps={sweeps} candidates={tops} pilots={pilots}", flush=True)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        for ref in (None, 16):
+            rep = ctx.tune_flips(gid, sweeps, tops, pilot_chains=pilots, seed_base=100, quality_tol=0.001,
+                                 reference_top_p=ref)
+            # the chosen candidate is within tolerance of the floor median
+            # (the reference ladder's, else the best) and the fastest such one
+            med, rate = rep["median_best"], rep["chain_steps_per_s"]
+            floor = med[ref] if ref else max(med.values())
+            ok = [t for t in tops if med[t] >= floor - 0.001 * abs(floor)]
+            assert rep["chosen_top_p"] in ok
+            assert rate[rep["chosen_top_p"]] == max(rate[t] for t in ok)
+        # each pilot is an ordinary run: the same seeds and ladder through the
+        # batch path give the same median
+        for t in tops:
+            phases, iters = R.flips_ladder(t, sweeps)
+            seeds = range(101, 101 + pilots)
+            job = make_job(csr, seeds, phases, iters)
+            ctx.batch_run([(gid, job)])
+            assert float(np.median(job.best_vals)) == med[t]

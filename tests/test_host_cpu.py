@@ -244,3 +244,14 @@ def test_footprint_grows_with_chains():
     a = R.footprint_bytes(20000, 80000, 64, 1000, 8)
     b = R.footprint_bytes(20000, 80000, 128, 1000, 8)
     assert b > a > 20000 * 64
+
+
+def test_flips_ladder_halves_to_one_and_splits_sweeps_evenly():
+    assert R.flips_ladder(128, 1000) == ([128, 64, 32, 16, 8, 4, 2, 1], [125] * 8)
+    assert R.flips_ladder(1, 7) == ([1], [7])
+    ph, it = R.flips_ladder(16, 1003)
+    assert ph == [16, 8, 4, 2, 1] and it == [201, 201, 201, 200, 200] and sum(it) == 1003
+    for bad, status in ((3, 1), (0, 1), (8192, 2)):
+        with pytest.raises(R.RfmfError) as e:
+            R.flips_ladder(bad, 10)
+        assert e.value.status == status

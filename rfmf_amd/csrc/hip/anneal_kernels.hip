@@ -315,12 +315,24 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
 
         const int nFull = n >> 3;
         const int nBlk = (n + 7) >> 3;
+        // Waves that own at least one block (all of them unless the chain has
+        // fewer blocks than threads).  The others skip the per-group
+        // reduction: their partial entries stay zero (empty).
+        const int busyWaves = nBlk >= NT ? NW : (nBlk + 63) >> 6;
 
         while (step < stepEnd) {
             if (step >= phaseStop && ph < nPhases - 1) {
                 while (ph < nPhases - 1 && uni(phaseEnd[ph]) <= step) ++ph;
                 phaseStop = uni(phaseEnd[ph]);
                 P = static_cast<int>(uni(phaseP[ph]));
+            }
+            // The next beta run, fetched now so that its latency hides behind
+            // SELECT (it is consumed after RESOLVE).
+            uint64_t nextEnd = runStop;
+            int32_t nextVal = beta;
+            if (run < nRuns - 1) {
+                nextEnd = static_cast<uint64_t>(uni(betaEnd[run + 1]));
+                nextVal = uni(betaVal[run + 1]);
             }
             const uint64_t s1 = step + 1;   // 1-based sweep id, the Philox counter
             const uint32_t sLo = static_cast<uint32_t>(s1);
@@ -388,6 +400,7 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                 if (nFull < nBlk && tid == (nFull % NT)) scoreBlock(nFull, true);
 
                 if constexpr (kLane) {
+                    if (wave >= busyWaves) return;
 #pragma unroll
                     for (int j = 0; j < 8; ++j) accV[j] = accV[j] >= 0 ? 8 * accV[j] + j : -1;
                     if (P >= 8) {
@@ -453,6 +466,7 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                 if (laneReduce) {
 #pragma unroll
                     for (int w = 0; w < NW; ++w) {
+                        if (w >= busyWaves) break;
                         const uint64_t k2 = part[w * pstride + g];
                         part[w * pstride + g] = 0ull;
                         if constexpr (kWide) {
@@ -487,9 +501,14 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
             // Beta of the next sweep, and its biased LUT (SELECT reads are done).
             int32_t betaNext = beta;
             if (runStop <= step + 1 && run < nRuns - 1) {
-                while (run < nRuns - 1 && static_cast<uint64_t>(uni(betaEnd[run])) <= step + 1) ++run;
-                runStop = static_cast<uint64_t>(uni(betaEnd[run]));
-                betaNext = uni(betaVal[run]);
+                ++run;
+                runStop = nextEnd;
+                betaNext = nextVal;
+                while (run < nRuns - 1 && runStop <= step + 1) {   // empty runs only
+                    ++run;
+                    runStop = static_cast<uint64_t>(uni(betaEnd[run]));
+                    betaNext = uni(betaVal[run]);
+                }
             }
             setLut(betaNext);
 
@@ -513,24 +532,39 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                     if (b >= 0) {
                         const uint32_t lo = offsets[b];
                         const uint32_t hi = offsets[b + 1];
-                        for (uint32_t i = lo + sub; i < hi; i += L) {
-                            const int2 e = edges[i];
-                            const int x = e.x;
-                            const int32_t w2 = 2 * e.y;
-                            const int gx = pow2 ? (x & (P - 1)) : (x % P);
-                            if (pk[gx] == x) {
-                                // x flips too: its coupling enters b's own
-                                // update, and, for an earlier group, the cut
-                                // delta as sequential group order sees it.
-                                const int32_t t = w2 * sb * ps[gx];
-                                own += t;
-                                if (gx < g) dsum -= t;
-                            } else {
-                                // delta(x) += s_x * (-2 w s_b); x's spin is stable.
-                                const bool up = (ldsWord<kBits>(wl, x) & F::kSpin) != 0;
-                                const int32_t d = up ? -w2 * sb : w2 * sb;
-                                ldsAddWord<kBits>(wl, x, static_cast<uint32_t>(d));
-                            }
+                        // Two edges per step: both rows entries are fetched,
+                        // then both neighbours' group slots and words are
+                        // read, then the updates are applied.  A non-winner's
+                        // spin bit does not change during FLIP, and a winner's
+                        // word read here is not used, so the reads may run
+                        // ahead of the atomics; the updates are integer sums.
+                        for (uint32_t i0 = lo + sub; i0 < hi; i0 += 2 * L) {
+                            const uint32_t i1 = i0 + static_cast<uint32_t>(L);
+                            const bool two = i1 < hi;
+                            const int2 e0 = edges[i0];
+                            const int2 e1 = two ? edges[i1] : make_int2(e0.x, 0);
+                            const int gx0 = pow2 ? (e0.x & (P - 1)) : (e0.x % P);
+                            const int gx1 = pow2 ? (e1.x & (P - 1)) : (e1.x % P);
+                            const int32_t pk0 = pk[gx0];
+                            const int32_t pk1 = pk[gx1];
+                            const uint32_t wd0 = ldsWord<kBits>(wl, e0.x);
+                            const uint32_t wd1 = ldsWord<kBits>(wl, e1.x);
+                            auto apply = [&](int x, int32_t w2, int gx, int32_t pkx, uint32_t wd) {
+                                if (pkx == x) {
+                                    // x flips too: its coupling enters b's own
+                                    // update, and, for an earlier group, the cut
+                                    // delta as sequential group order sees it.
+                                    const int32_t t = w2 * sb * ps[gx];
+                                    own += t;
+                                    if (gx < g) dsum -= t;
+                                } else {
+                                    // delta(x) += s_x * (-2 w s_b); x's spin is stable.
+                                    const int32_t d = (wd & F::kSpin) ? -w2 * sb : w2 * sb;
+                                    ldsAddWord<kBits>(wl, x, static_cast<uint32_t>(d));
+                                }
+                            };
+                            apply(e0.x, 2 * e0.y, gx0, pk0, wd0);
+                            if (two) apply(e1.x, 2 * e1.y, gx1, pk1, wd1);
                         }
                     }
 #pragma unroll

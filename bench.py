@@ -71,7 +71,7 @@ def main() -> int:
     ap.add_argument("--cpu-sweeps", type=int, default=64000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (counter calibration) and exit")
-    ap.add_argument("--config", default="torus", choices=["torus", "er800", "sparse", "batch", "serve"],
+    ap.add_argument("--config", default="torus", choices=["torus", "er800", "sparse", "batch", "serve", "gset"],
                     help="torus: the headline plain run (default); the others are the other workloads")
     ap.add_argument("--threads", type=int, default=0, help="threads per chain (0: autotune)")
     ap.add_argument("--qps", type=float, default=20.0, help="serve: Poisson arrival rate")
@@ -81,7 +81,7 @@ def main() -> int:
 
     if args.config != "torus" and not args.copy_probe:
         return {"er800": bench_er800, "sparse": bench_sparse, "batch": bench_batch,
-                "serve": bench_serve}[args.config](args)
+                "serve": bench_serve, "gset": bench_gset}[args.config](args)
 
     if args.copy_probe:
         import rfmf_amd as R
@@ -384,6 +384,45 @@ def bench_serve(args) -> int:
     log(f"[serve] {out}")
     _emit(out)
     return 0 if same else 1
+
+
+# Reference runs on Tesla V100-SXM2, 80 chains, 8e7 sweeps per chain:
+# chain-sweeps/s from results/V100-SXM2-10M/results.csv:3,4,10,13.
+V100_SXM2 = {"G61": 7.40e6, "G62": 8.09e6, "G70": 7.31e6, "G81": 5.32e6}
+
+
+def bench_gset(args) -> int:
+    """The reference's own benchmark shape on the Gset instances held here:
+    80 chains (t0 in {35, 50, 70, 100, 140, 200, 280, 400} x 10 replicas),
+    ladder 128..1 with equal sweeps per phase, log schedule, one launch per
+    instance with the per-phase autotuned geometry; chain-sweeps/s next to
+    the reference's published V100-SXM2 figure."""
+    import rfmf_amd as R
+    sweeps = args.steps if args.steps != 40000 else 100_000
+    iters = [sweeps // 8] * 8
+    t0s = [float(t) for t in (35, 50, 70, 100, 140, 200, 280, 400) for _ in range(10)]
+    rc = 0
+    for inst in ("G61", "G62", "G70", "G81"):
+        csr, _ = R.read_instance_from_file(str(ROOT / "tests" / "golden" / "gset" / inst))
+        seeds = list(range(1, 81))
+        job = R.Job(csr, np.stack([R.generate_random_spins(s, csr.n) for s in seeds]),
+                    [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds],
+                    R.build_beta_run_tables(t0s, sum(iters)), LADDER, iters)
+        with R.Context(0) as ctx:
+            gid = ctx.upload_graph(csr)
+            ms = ctx.batch_run([(gid, job)], autotune=args.threads == 0, threads=args.threads)
+            geo = ctx.last_launch()
+        ok = all(R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c] for c in range(80))
+        rc |= 0 if ok else 1
+        rate = 80 * sum(iters) / (ms / 1e3)
+        log(f"[gset] {inst}: n={csr.n} {ms:.1f} ms {rate:.3e} chain-sweeps/s {ms * 1e3 / sum(iters):.2f} us/sweep "
+            f"(V100-SXM2 {V100_SXM2[inst]:.2e}: {rate / V100_SXM2[inst]:.2f}x) last phase {geo['threads_per_chain']} "
+            f"threads")
+        _emit({"metric": "chain-sweeps/sec", "config": f"gset_{inst}_80chains", "n": csr.n, "sweeps": sum(iters),
+               "value": rate, "us_per_sweep": ms * 1e3 / sum(iters), "reference_v100_sxm2": V100_SXM2[inst],
+               "ratio_vs_reference": rate / V100_SXM2[inst], "best_cut_max": int(job.best_vals.max()),
+               "cuts_recomputed": bool(ok)})
+    return rc
 
 
 def run_two_slices(ctx, gid, job, warm: int, timed: int):

@@ -354,6 +354,8 @@ std::unique_ptr<ResidentJob> Context::placeJob(const HostJob& hj, const DevGraph
     job->chains = C;
     job->totalSteps = hj.totalSteps;
     job->maxP = hj.maxP;
+    job->hPhaseEnd = hj.phaseEnd;
+    job->hPhaseP = hj.phaseP;
     job->bits = hj.bits;
     job->mode = hj.mode;
 
@@ -601,7 +603,8 @@ float Context::waitSlice(SliceHandle& h) {
     return ms;
 }
 
-int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* trialMs) {
+int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* trialMs, int keyP,
+                      uint64_t budget) {
     // Shape of the launch: widest word and score, largest n and P, unfinished chains.
     int bits = 8, mode = 0, maxP = 1;
     int64_t maxN = 0, chains = 0;
@@ -617,14 +620,15 @@ int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& 
             minLeft = std::min(minLeft, j->totalSteps - j->stepsDone[c]);
         }
     if (chains == 0) return 0;
-    const uint64_t key = tuneKey(bits, mode, maxN, chains, maxP);
+    const uint64_t key = tuneKey(bits, mode, maxN, chains, keyP > 0 ? keyP : maxP);
     const auto hit = tuned_.find(key);
     last_.tuned = true;
     if (hit != tuned_.end()) return hit->second;
     // Each candidate runs one short slice of the live run: geometry does not
     // change any chain's arithmetic, so the trial sweeps are part of the
-    // result, and the trials together use at most a fifth of the sweeps.
-    const uint64_t trial = std::max<uint64_t>(1, std::min<uint64_t>(256, minLeft / 20));
+    // result, and the trials together use at most a tenth of the sweeps.
+    const uint64_t span = budget > 0 ? std::min(budget, minLeft) : minLeft;
+    const uint64_t trial = std::max<uint64_t>(1, std::min<uint64_t>(256, span / 40));
     int best = 0;
     float bestPerSweep = 0.0f;
     for (int i = 0; i < 4; ++i) {
@@ -647,6 +651,33 @@ float Context::runToCompletion(const std::vector<ResidentJob*>& jobs, const Laun
     float total = 0.0f;
     LaunchKnobs knobs = knobsIn;
     last_ = LaunchReport{};
+    auto pending = [&] {
+        for (const ResidentJob* j : jobs)
+            for (uint64_t s : j->stepsDone)
+                if (s < j->totalSteps) return true;
+        return false;
+    };
+    if (knobs.autotune && knobs.threads == 0 && !knobs.hipGraph && lockstep(jobs)) {
+        // Every chain is at the same sweep of the same phase list: tune the
+        // geometry per phase (the best tile changes with the group count)
+        // and cut slices at the phase ends.
+        const ResidentJob* j0 = jobs.front();
+        for (size_t ph = 0; ph < j0->hPhaseEnd.size() && pending(); ++ph) {
+            const uint64_t at = j0->stepsDone.front();
+            const uint64_t end = j0->hPhaseEnd[ph];
+            if (end <= at) continue;
+            LaunchKnobs k = knobsIn;
+            k.threads = autotune(jobs, k, nullptr, static_cast<int>(j0->hPhaseP[ph]), end - at);
+            while (pending() && j0->stepsDone.front() < end) {
+                const uint64_t left = end - j0->stepsDone.front();
+                k.sliceSweeps = static_cast<int64_t>(knobsIn.sliceSweeps > 0
+                                                         ? std::min<uint64_t>(left, knobsIn.sliceSweeps)
+                                                         : left);
+                total += runSlice(jobs, k);
+            }
+        }
+        return total;
+    }
     if (knobs.autotune && knobs.threads == 0) knobs.threads = autotune(jobs, knobs, nullptr);
     if (knobs.hipGraph && knobs.sliceSweeps > 0) {
         // Every chain starts together here, so ceil(longest / slice) replays
@@ -661,14 +692,20 @@ float Context::runToCompletion(const std::vector<ResidentJob*>& jobs, const Laun
         }
         return total;
     }
-    for (;;) {
-        bool pending = false;
-        for (const ResidentJob* j : jobs)
-            for (uint64_t s : j->stepsDone) pending |= s < j->totalSteps;
-        if (!pending) break;
-        total += runSlice(jobs, knobs);
-    }
+    while (pending()) total += runSlice(jobs, knobs);
     return total;
+}
+
+bool Context::lockstep(const std::vector<ResidentJob*>& jobs) {
+    if (jobs.empty() || jobs.front()->stepsDone.empty()) return false;
+    const ResidentJob* j0 = jobs.front();
+    const uint64_t at = j0->stepsDone.front();
+    for (const ResidentJob* j : jobs) {
+        if (j->hPhaseEnd != j0->hPhaseEnd || j->hPhaseP != j0->hPhaseP) return false;
+        for (uint64_t s : j->stepsDone)
+            if (s != at) return false;
+    }
+    return true;
 }
 
 void Context::collect(const ResidentJob& job, const JobSpec& spec, hipStream_t stream) {

@@ -147,6 +147,8 @@ struct ResidentJob {
     uint64_t totalSteps = 0;   // executed sweeps per chain
     DevBuf words, best, states, runEnd, runVal, phaseEnd, phaseP;
     int64_t wordStride = 0, bestStride = 0;
+    std::vector<uint64_t> hPhaseEnd;     // executed phases (host copy): cumulative end sweep
+    std::vector<uint32_t> hPhaseP;       //   and group count
     std::vector<dev::ChainItem> items;   // one per chain (device pointers filled)
     std::vector<uint64_t> stepsDone;     // host mirror of each chain's step after every enqueued slice
     int64_t footprint = 0;
@@ -216,8 +218,12 @@ public:
     };
     SliceHandle launchSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs);
     float waitSlice(SliceHandle& h);
-    // Runs jobs to completion (slices repeated) and returns kernel ms.
+    // Runs jobs to completion (slices repeated) and returns kernel ms.  With
+    // autotuning, chains that share one phase list and sweep are tuned per
+    // phase.
     float runToCompletion(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs);
+    // Every chain of `jobs` has the same phase list and the same sweep count.
+    static bool lockstep(const std::vector<ResidentJob*>& jobs);
 
     // A fixed set of chains whose slice launches are captured once into a
     // hipGraph (item arrays uploaded once) and replayed per slice.
@@ -233,7 +239,11 @@ public:
                                                  int gridCap, int threads);
     // Autotuner: times each thread-count candidate on one short slice of the
     // live run (output-invariant), caches and returns the fastest.
-    int autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* trialMs);
+    // `keyP` (0: the jobs' largest P) and `budget` (0: the shortest remaining
+    // run) narrow the tuning to one phase: the cache key uses keyP and the
+    // trials together stay within a tenth of the budget.
+    int autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* trialMs, int keyP = 0,
+                 uint64_t budget = 0);
     // Replays the captured slice `times` times; returns kernel-side ms.
     float replaySlices(GraphedSlices& g, int times);
     // Copies a finished job's results to the spec's outputs.

@@ -173,11 +173,138 @@ __device__ __forceinline__ void keepBetter(S& s, int32_t& v, S s2, int32_t v2) {
     v = take ? v2 : v;
 }
 
-__device__ __forceinline__ int32_t shflXor(int32_t x, int m) { return __shfl_xor(x, m, 64); }
-__device__ __forceinline__ int64_t shflXor(int64_t x, int m) {
-    const int32_t lo = __shfl_xor(static_cast<int32_t>(x), m, 64);
-    const int32_t hi = __shfl_xor(static_cast<int32_t>(x >> 32), m, 64);
+// Cross-lane exchanges on the VALU (no LDS crossbar).  laneXor<d> returns
+// the value of lane l ^ d for d = 1, 2 (DPP quad_perm), 4 (row_shl / row_shr
+// by 4, chosen by the lane's bit 2) and 8 (row_ror by 8 within a 16-lane
+// row); for 16 and 32, v_permlane16_swap / v_permlane32_swap pair each lane
+// with lane l ^ 16 / l ^ 32 and return both values.
+template <int kDist>
+__device__ __forceinline__ int32_t rowRor(int32_t x) {
+    static_assert(kDist >= 1 && kDist < 16, "row rotation distance");
+    return __builtin_amdgcn_update_dpp(0, x, 0x120 + kDist, 0xF, 0xF, false);
+}
+template <int kDist>
+__device__ __forceinline__ int32_t laneXor(int32_t x, int lane) {
+    if constexpr (kDist == 1) {
+        return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    } else if constexpr (kDist == 2) {
+        return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    } else if constexpr (kDist == 4) {
+        const int32_t up = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xF, 0xF, false);     // row_shl:4 (lane + 4)
+        const int32_t down = __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);   // row_shr:4 (lane - 4)
+        return (lane & 4) ? down : up;
+    } else {
+        static_assert(kDist == 8, "lane xor distance");
+        return rowRor<8>(x);
+    }
+}
+template <int kDist>
+__device__ __forceinline__ void swapPair(int32_t x, int32_t& a, int32_t& b) {
+    static_assert(kDist == 16 || kDist == 32, "row/half swap distance");
+    const uint32_t u = static_cast<uint32_t>(x);
+    if constexpr (kDist == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+        a = static_cast<int32_t>(r[0]);
+        b = static_cast<int32_t>(r[1]);
+    } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+        a = static_cast<int32_t>(r[0]);
+        b = static_cast<int32_t>(r[1]);
+    }
+}
+// The same swaps on two registers: with (a, b), lanes whose kDist bit is 0
+// get (own a, partner's a) and lanes whose bit is 1 get (partner's b, own b).
+template <int kDist>
+__device__ __forceinline__ void swapRows(int32_t a, int32_t b, int32_t& x, int32_t& y) {
+    static_assert(kDist == 16 || kDist == 32, "row/half swap distance");
+    if constexpr (kDist == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(static_cast<uint32_t>(a), static_cast<uint32_t>(b), false,
+                                                         false);
+        x = static_cast<int32_t>(r[0]);
+        y = static_cast<int32_t>(r[1]);
+    } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(static_cast<uint32_t>(a), static_cast<uint32_t>(b), false,
+                                                         false);
+        x = static_cast<int32_t>(r[0]);
+        y = static_cast<int32_t>(r[1]);
+    }
+}
+__device__ __forceinline__ int32_t lo32(int64_t x) { return static_cast<int32_t>(x); }
+__device__ __forceinline__ int32_t hi32(int64_t x) { return static_cast<int32_t>(x >> 32); }
+__device__ __forceinline__ int64_t join64(int32_t lo, int32_t hi) {
     return (static_cast<int64_t>(hi) << 32) | static_cast<uint32_t>(lo);
+}
+
+// One reduce-scatter step of the per-group argmax for a pair of slots (a, b)
+// and lane distance kDist in {8, 16, 32}: lanes whose kDist bit is 0 keep slot
+// a, the others slot b, each combined with its partner's copy; the result is
+// left in (sa, va).  Halving the held slots per step avoids reducing every
+// slot in every lane.
+template <int kDist, class S>
+__device__ __forceinline__ void scatterStep(S& sa, int32_t& va, const S& sb, const int32_t& vb, int lane) {
+    if constexpr (kDist == 8) {
+        const bool upper = (lane & 8) != 0;
+        S keep = upper ? sb : sa;
+        int32_t kv = upper ? vb : va;
+        const S send = upper ? sa : sb;
+        const int32_t sv = upper ? va : vb;
+        S recv;
+        if constexpr (sizeof(S) == 8)
+            recv = join64(rowRor<8>(lo32(send)), rowRor<8>(hi32(send)));
+        else
+            recv = rowRor<8>(send);
+        keepBetter(keep, kv, recv, rowRor<8>(sv));
+        sa = keep;
+        va = kv;
+    } else {
+        S x, y;
+        int32_t xv, yv;
+        if constexpr (sizeof(S) == 8) {
+            int32_t xl, yl, xh, yh;
+            swapRows<kDist>(lo32(sa), lo32(sb), xl, yl);
+            swapRows<kDist>(hi32(sa), hi32(sb), xh, yh);
+            x = join64(xl, xh);
+            y = join64(yl, yh);
+        } else {
+            swapRows<kDist>(sa, sb, x, y);
+        }
+        swapRows<kDist>(va, vb, xv, yv);
+        keepBetter(x, xv, y, yv);
+        sa = x;
+        va = xv;
+    }
+}
+
+// One butterfly step of the per-group argmax across lanes: every lane keeps
+// the better (score, id) of itself and lane l ^ kDist.  Run for kDist = Q,
+// 2Q, ..., it leaves in every lane the best over the lanes that agree with it
+// in every other bit (the order of a lexicographic max does not matter).
+template <int kDist, class S>
+__device__ __forceinline__ void foldLanes(S& s, int32_t& v, int lane) {
+    if constexpr (kDist < 16) {
+        S s2;
+        if constexpr (sizeof(S) == 8)
+            s2 = join64(laneXor<kDist>(lo32(s), lane), laneXor<kDist>(hi32(s), lane));
+        else
+            s2 = laneXor<kDist>(s, lane);
+        keepBetter(s, v, s2, laneXor<kDist>(v, lane));
+    } else {
+        S sa, sb;
+        int32_t va, vb;
+        if constexpr (sizeof(S) == 8) {
+            int32_t la, lb, ha, hb;
+            swapPair<kDist>(lo32(s), la, lb);
+            swapPair<kDist>(hi32(s), ha, hb);
+            sa = join64(la, ha);
+            sb = join64(lb, hb);
+        } else {
+            swapPair<kDist>(s, sa, sb);
+        }
+        swapPair<kDist>(v, va, vb);
+        keepBetter(sa, va, sb, vb);
+        s = sa;
+        v = va;
+    }
 }
 
 __device__ __forceinline__ int64_t waveSum64(int64_t x) {
@@ -404,18 +531,42 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
 #pragma unroll
                     for (int j = 0; j < 8; ++j) accV[j] = accV[j] >= 0 ? 8 * accV[j] + j : -1;
                     if (P >= 8) {
-                        const int Q = P >> 3;   // lanes l and l' share groups iff l = l' (mod Q)
-                        for (int m = Q; m < 64; m <<= 1) {
+                        // Lanes l and l' share groups iff l = l' (mod Q); slot j
+                        // of lane l feeds group 8 (l mod Q) + j.  Reduce-scatter
+                        // over the lane distances 32, 16, 8 that are >= Q: each
+                        // step halves the slots a lane holds (held slot k is
+                        // slot base + k, base from the lane's bits 5, 4, 3),
+                        // then plain folds over the distances 4, 2, 1 >= Q.
+                        const int Q = P >> 3;
+                        int held = 8, base = 0;
+                        if (Q <= 32) {
 #pragma unroll
-                            for (int j = 0; j < 8; ++j) {
-                                const Score s2 = shflXor(accS[j], m);
-                                const int32_t v2 = shflXor(accV[j], m);
-                                keepBetter(accS[j], accV[j], s2, v2);
-                            }
+                            for (int j = 0; j < 4; ++j) scatterStep<32>(accS[j], accV[j], accS[j + 4], accV[j + 4], lane);
+                            held = 4;
+                            base += (lane & 32) ? 4 : 0;
                         }
-                        if (lane < Q) {
+                        if (Q <= 16) {
 #pragma unroll
-                            for (int j = 0; j < 8; ++j) putPartial(wave * pstride + 8 * lane + j, accS[j], accV[j]);
+                            for (int j = 0; j < 2; ++j) scatterStep<16>(accS[j], accV[j], accS[j + 2], accV[j + 2], lane);
+                            held = 2;
+                            base += (lane & 16) ? 2 : 0;
+                        }
+                        if (Q <= 8) {
+                            scatterStep<8>(accS[0], accV[0], accS[1], accV[1], lane);
+                            held = 1;
+                            base += (lane & 8) ? 1 : 0;
+                        }
+                        if (Q <= 4) foldLanes<4>(accS[0], accV[0], lane);
+                        if (Q <= 2) foldLanes<2>(accS[0], accV[0], lane);
+                        if (Q <= 1) foldLanes<1>(accS[0], accV[0], lane);
+                        // After the folds the lanes differing only in bits
+                        // below 3 and at or above log2 Q hold equal values:
+                        // one of them writes.
+                        if ((lane & 7) < Q) {
+                            const int row = wave * pstride + 8 * (lane & (Q - 1)) + base;
+#pragma unroll
+                            for (int k = 0; k < 8; ++k)
+                                if (k < held) putPartial(row + k, accS[k], accV[k]);
                         }
                     } else {
                         // P in {1, 2, 4}: slot j feeds group j mod P.
@@ -428,15 +579,18 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                             for (int j = 0; j < 2; ++j) keepBetter(accS[j], accV[j], accS[j + 2], accV[j + 2]);
                         }
                         if (P == 1) keepBetter(accS[0], accV[0], accS[1], accV[1]);
+                        auto fold4 = [&](auto distTag) {
+                            constexpr int D = decltype(distTag)::value;
 #pragma unroll
-                        for (int m = 1; m < 64; m <<= 1) {
-#pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                const Score s2 = shflXor(accS[j], m);
-                                const int32_t v2 = shflXor(accV[j], m);
-                                keepBetter(accS[j], accV[j], s2, v2);
-                            }
-                        }
+                            for (int j = 0; j < 4; ++j)
+                                if (j < P) foldLanes<D>(accS[j], accV[j], lane);
+                        };
+                        fold4(std::integral_constant<int, 1>{});
+                        fold4(std::integral_constant<int, 2>{});
+                        fold4(std::integral_constant<int, 4>{});
+                        fold4(std::integral_constant<int, 8>{});
+                        fold4(std::integral_constant<int, 16>{});
+                        fold4(std::integral_constant<int, 32>{});
                         if (lane == 0) {
 #pragma unroll
                             for (int j = 0; j < 4; ++j)

@@ -107,6 +107,14 @@ def test_wide_words_u32():
     _case("u32-words", csr, [3, 4], beta, phases, iters)
 
 
+def test_wide_ladder_register_argmax(gset):
+    # P = 512 and 256 keep the register argmax (lanes congruent mod 64 / 32
+    # share a group), next to the 128..1 steps already covered
+    csr = gset("G62")
+    phases, iters = [512, 256, 128, 1], [60, 60, 60, 60]
+    _case("G62-P512", csr, [3, 4], _log_tables([40.0, 60.0], iters), phases, iters)
+
+
 def test_p_zero_phase_skipped(gset):
     csr = gset("G11")
     phases, iters = [8, 0, 4], [100, 50, 100]

@@ -579,23 +579,28 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                             for (int j = 0; j < 2; ++j) keepBetter(accS[j], accV[j], accS[j + 2], accV[j + 2]);
                         }
                         if (P == 1) keepBetter(accS[0], accV[0], accS[1], accV[1]);
-                        auto fold4 = [&](auto distTag) {
-                            constexpr int D = decltype(distTag)::value;
-#pragma unroll
-                            for (int j = 0; j < 4; ++j)
-                                if (j < P) foldLanes<D>(accS[j], accV[j], lane);
-                        };
-                        fold4(std::integral_constant<int, 1>{});
-                        fold4(std::integral_constant<int, 2>{});
-                        fold4(std::integral_constant<int, 4>{});
-                        fold4(std::integral_constant<int, 8>{});
-                        fold4(std::integral_constant<int, 16>{});
-                        fold4(std::integral_constant<int, 32>{});
-                        if (lane == 0) {
-#pragma unroll
-                            for (int j = 0; j < 4; ++j)
-                                if (j < P) putPartial(wave * pstride + j, accS[j], accV[j]);
+                        // Every lane now holds slots 0..P-1 (group j); the
+                        // same reduce-scatter leaves one group per lane (base),
+                        // then folds over the remaining lane bits.
+                        int base = 0;
+                        if (P == 4) {
+                            scatterStep<32>(accS[0], accV[0], accS[2], accV[2], lane);
+                            scatterStep<32>(accS[1], accV[1], accS[3], accV[3], lane);
+                            base += (lane & 32) ? 2 : 0;
+                            scatterStep<16>(accS[0], accV[0], accS[1], accV[1], lane);
+                            base += (lane & 16) ? 1 : 0;
+                        } else if (P == 2) {
+                            scatterStep<32>(accS[0], accV[0], accS[1], accV[1], lane);
+                            base += (lane & 32) ? 1 : 0;
                         }
+                        if (P == 1) foldLanes<32>(accS[0], accV[0], lane);
+                        if (P <= 2) foldLanes<16>(accS[0], accV[0], lane);
+                        foldLanes<8>(accS[0], accV[0], lane);
+                        foldLanes<4>(accS[0], accV[0], lane);
+                        foldLanes<2>(accS[0], accV[0], lane);
+                        foldLanes<1>(accS[0], accV[0], lane);
+                        // lanes 0, 16, 32, 48 (P = 4), 0 and 32 (P = 2), 0 (P = 1)
+                        if ((lane & (63 >> (P >> 1))) == 0) putPartial(wave * pstride + base, accS[0], accV[0]);
                     }
                 }
             };

@@ -222,13 +222,15 @@ rfmf_status rfmf_ctx_last_launch(rfmf_ctx* ctx, int32_t* threads, int32_t* grid,
 rfmf_status rfmf_batch_run(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs,
                            const rfmf_tuning* tuning, float* kernel_ms);
 
-/* rfmf_batch_run cut into timed windows: launch i advances every chain to
+/* rfmf_batch_run cut into timed windows: window i advances every chain to
  * sweep min(total, step_bounds[i]) (bounds ascending) and window_ms[i]
- * receives its kernel-only time; after the last window the chains run to
- * completion untimed.  Used to time a window of a run with the state already
- * resident (e.g. after warmup sweeps). */
+ * receives its kernel-only time (every launch of the window, autotuner trials
+ * included); after the last window the chains run to completion untimed.
+ * Used to time a window of a run with the state already resident (e.g. after
+ * warmup sweeps).  `tuning` may be NULL (defaults). */
 rfmf_status rfmf_batch_run_windows(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs,
-                                   const int64_t* step_bounds, int64_t n_windows, float* window_ms);
+                                   const int64_t* step_bounds, int64_t n_windows, const rfmf_tuning* tuning,
+                                   float* window_ms);
 
 /* Step-wise control of a batch: admit jobs once (chain state resident in
  * HBM), advance every chain by `sweeps` per call (one persistent launch per

@@ -399,13 +399,15 @@ class Context:
                 "median_best": {int(t): rep.median_best[i] for i, t in enumerate(tops)},
                 "chain_steps_per_s": {int(t): rep.chain_steps_per_s[i] for i, t in enumerate(tops)}}
 
-    def batch_run_windows(self, jobs: list[tuple[int, Job]], bounds: list[int]) -> list[float]:
-        """Runs jobs with one timed launch per window [bounds[i-1], bounds[i])
-        of sweeps, then to completion; returns the windows' kernel ms."""
+    def batch_run_windows(self, jobs: list[tuple[int, Job]], bounds: list[int], threads: int = 0,
+                          autotune: bool = False) -> list[float]:
+        """Runs jobs window by window ([bounds[i-1], bounds[i]) sweeps), then
+        to completion; returns each window's kernel ms (all its launches)."""
         arr = (_capi.Job * len(jobs))(*[j.struct(gid) for gid, j in jobs])
         b = np.ascontiguousarray(bounds, dtype=np.int64)
         ms = (C.c_float * max(1, b.size))()
-        check(lib().rfmf_batch_run_windows(self._h, arr, len(jobs), _ptr(b, C.c_int64), b.size, ms))
+        tun = _tuning(0, 0, False, threads, autotune)
+        check(lib().rfmf_batch_run_windows(self._h, arr, len(jobs), _ptr(b, C.c_int64), b.size, C.byref(tun), ms))
         return [ms[i] for i in range(b.size)]
 
 

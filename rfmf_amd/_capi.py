@@ -100,7 +100,8 @@ SIGNATURES = {
     "rfmf_ctx_copy_probe": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, f32p]),
     "rfmf_ctx_last_launch": (C.c_int, [C.c_void_p, i32p, i32p, i32p, i32p, f32p]),
     "rfmf_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int64, C.POINTER(Tuning), f32p]),
-    "rfmf_batch_run_windows": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int64, i64p, C.c_int64, f32p]),
+    "rfmf_batch_run_windows": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int64, i64p, C.c_int64,
+                                         C.POINTER(Tuning), f32p]),
     "rfmf_run_create": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int64, C.POINTER(C.c_void_p)]),
     "rfmf_run_advance": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Tuning), f32p, i32p]),
     "rfmf_run_collect": (C.c_int, [C.c_void_p]),

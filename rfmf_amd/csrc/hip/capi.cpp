@@ -546,7 +546,8 @@ void rfmf_run_destroy(rfmf_run* run) {
 }
 
 rfmf_status rfmf_batch_run_windows(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t n_jobs,
-                                   const int64_t* step_bounds, int64_t n_windows, float* window_ms) {
+                                   const int64_t* step_bounds, int64_t n_windows, const rfmf_tuning* tuning,
+                                   float* window_ms) {
     return guarded([&] {
         if (!ctx || n_jobs <= 0 || !jobs || n_windows < 0 || (n_windows > 0 && (!step_bounds || !window_ms)))
             throw Error(Status::InvalidArg, "invalid batch or windows");
@@ -563,16 +564,12 @@ rfmf_status rfmf_batch_run_windows(rfmf_ctx* ctx, const rfmf_job* jobs, int64_t 
             resident.push_back(c.admit(s, c.stream()));
             ptrs.push_back(resident.back().get());
         }
-        // Every chain of a fresh batch starts at sweep 0, so window i is one
-        // slice of (bound_i - bound_{i-1}) sweeps.
-        int64_t at = 0;
-        for (int64_t i = 0; i < n_windows; ++i) {
-            rfmf::LaunchKnobs k;
-            k.sliceSweeps = step_bounds[i] - at;
-            window_ms[i] = k.sliceSweeps > 0 ? c.runSlice(ptrs, k) : 0.0f;
-            at = step_bounds[i];
-        }
-        c.runToCompletion(ptrs, rfmf::LaunchKnobs{});
+        // Every chain of a fresh batch starts at sweep 0 and each slice
+        // advances all of them alike, so every window ends at its bound.
+        const rfmf::LaunchKnobs k = knobsFrom(tuning);
+        for (int64_t i = 0; i < n_windows; ++i)
+            window_ms[i] = c.runUntil(ptrs, k, static_cast<uint64_t>(step_bounds[i]));
+        c.runToCompletion(ptrs, k);
         for (size_t i = 0; i < specs.size(); ++i) c.collect(*resident[i], specs[i], c.stream());
     });
 }

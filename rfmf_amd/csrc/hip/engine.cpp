@@ -647,39 +647,54 @@ int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& 
     return best;
 }
 
-float Context::runToCompletion(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobsIn) {
+float Context::runToCompletion(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs) {
+    return runUntil(jobs, knobs, ~uint64_t{0});
+}
+
+float Context::runUntil(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobsIn, uint64_t until) {
     float total = 0.0f;
     LaunchKnobs knobs = knobsIn;
     last_ = LaunchReport{};
-    auto pending = [&] {
-        for (const ResidentJob* j : jobs)
+    // The smallest sweep of a chain that still has sweeps to run before `until`.
+    auto lowest = [&](uint64_t& at) {
+        bool any = false;
+        at = ~uint64_t{0};
+        for (const ResidentJob* j : jobs) {
+            const uint64_t lim = std::min(j->totalSteps, until);
             for (uint64_t s : j->stepsDone)
-                if (s < j->totalSteps) return true;
-        return false;
+                if (s < lim) {
+                    any = true;
+                    at = std::min(at, s);
+                }
+        }
+        return any;
     };
+    auto sliceTo = [&](uint64_t at, uint64_t stop) -> int64_t {
+        if (stop == ~uint64_t{0}) return knobsIn.sliceSweeps;   // 0: the rest in one launch
+        const uint64_t room = stop - at;
+        return static_cast<int64_t>(knobsIn.sliceSweeps > 0 ? std::min<uint64_t>(room, knobsIn.sliceSweeps)
+                                                             : room);
+    };
+    uint64_t at = 0;
     if (knobs.autotune && knobs.threads == 0 && !knobs.hipGraph && lockstep(jobs)) {
         // Every chain is at the same sweep of the same phase list: tune the
         // geometry per phase (the best tile changes with the group count)
         // and cut slices at the phase ends.
         const ResidentJob* j0 = jobs.front();
-        for (size_t ph = 0; ph < j0->hPhaseEnd.size() && pending(); ++ph) {
-            const uint64_t at = j0->stepsDone.front();
-            const uint64_t end = j0->hPhaseEnd[ph];
+        for (size_t ph = 0; ph < j0->hPhaseEnd.size() && lowest(at); ++ph) {
+            const uint64_t end = std::min(j0->hPhaseEnd[ph], until);
             if (end <= at) continue;
             LaunchKnobs k = knobsIn;
             k.threads = autotune(jobs, k, nullptr, static_cast<int>(j0->hPhaseP[ph]), end - at);
-            while (pending() && j0->stepsDone.front() < end) {
-                const uint64_t left = end - j0->stepsDone.front();
-                k.sliceSweeps = static_cast<int64_t>(knobsIn.sliceSweeps > 0
-                                                         ? std::min<uint64_t>(left, knobsIn.sliceSweeps)
-                                                         : left);
+            while (lowest(at) && at < end) {
+                k.sliceSweeps = sliceTo(at, end);
                 total += runSlice(jobs, k);
             }
         }
         return total;
     }
     if (knobs.autotune && knobs.threads == 0) knobs.threads = autotune(jobs, knobs, nullptr);
-    if (knobs.hipGraph && knobs.sliceSweeps > 0) {
+    if (knobs.hipGraph && knobs.sliceSweeps > 0 && until == ~uint64_t{0}) {
         // Every chain starts together here, so ceil(longest / slice) replays
         // of one captured slice finish them all.
         uint64_t longest = 0;
@@ -692,7 +707,11 @@ float Context::runToCompletion(const std::vector<ResidentJob*>& jobs, const Laun
         }
         return total;
     }
-    while (pending()) total += runSlice(jobs, knobs);
+    while (lowest(at)) {
+        LaunchKnobs k = knobs;
+        k.sliceSweeps = sliceTo(at, until);
+        total += runSlice(jobs, k);
+    }
     return total;
 }
 

@@ -222,6 +222,9 @@ public:
     // autotuning, chains that share one phase list and sweep are tuned per
     // phase.
     float runToCompletion(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs);
+    // The same, stopping every chain at sweep `until` (chains that start a
+    // window together stay together: each slice advances all of them alike).
+    float runUntil(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, uint64_t until);
     // Every chain of `jobs` has the same phase list and the same sweep count.
     static bool lockstep(const std::vector<ResidentJob*>& jobs);
 

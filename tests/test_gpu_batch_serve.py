@@ -206,3 +206,14 @@ def test_flips_tuner_gate_and_pilots_are_ordinary_runs(gset):
             job = make_job(csr, seeds, phases, iters)
             ctx.batch_run([(gid, job)])
             assert float(np.median(job.best_vals)) == med[t]
+
+
+def test_windows_with_per_phase_autotune_equal_one_shot(graphs):
+    csr = graphs["torus"]
+    job = make_job(csr, [51, 52, 53, 54], LADDER, [40] * 8)
+    print("[windows] torus, bounds 50 / 170, per-phase autotune", flush=True)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ms = ctx.batch_run_windows([(gid, job)], [50, 170], autotune=True)
+    assert len(ms) == 2 and all(m > 0 for m in ms)
+    same(job, one_shot(job), "windows")

@@ -603,7 +603,7 @@ float Context::waitSlice(SliceHandle& h) {
     return ms;
 }
 
-int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* trialMs, int keyP,
+int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* spentMs, int keyP,
                       uint64_t budget) {
     // Shape of the launch: widest word and score, largest n and P, unfinished chains.
     int bits = 8, mode = 0, maxP = 1;
@@ -626,9 +626,12 @@ int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& 
     if (hit != tuned_.end()) return hit->second;
     // Each candidate runs one short slice of the live run: geometry does not
     // change any chain's arithmetic, so the trial sweeps are part of the
-    // result, and the trials together use at most a tenth of the sweeps.
+    // result (and their time is part of the caller's total), and the trials
+    // together use at most a twenty-fifth of the sweeps.  Candidates go from
+    // the smallest tile up; once one is a quarter slower than the best so far
+    // the larger ones are not tried.
     const uint64_t span = budget > 0 ? std::min(budget, minLeft) : minLeft;
-    const uint64_t trial = std::max<uint64_t>(1, std::min<uint64_t>(256, span / 40));
+    const uint64_t trial = std::max<uint64_t>(1, std::min<uint64_t>(128, span / 100));
     int best = 0;
     float bestPerSweep = 0.0f;
     for (int i = 0; i < 4; ++i) {
@@ -638,10 +641,15 @@ int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& 
         k.threads = t;
         k.sliceSweeps = static_cast<int64_t>(trial);
         const float ms = runSlice(jobs, k);
+        if (spentMs) *spentMs += ms;
         const float per = ms / static_cast<float>(trial);
-        if (trialMs) trialMs[i] = per;
         last_.trialMsPerSweep[i] = per;
-        if (best == 0 || per < bestPerSweep) { best = t; bestPerSweep = per; }
+        if (best == 0 || per < bestPerSweep) {
+            best = t;
+            bestPerSweep = per;
+        } else if (per > 1.25f * bestPerSweep) {
+            break;
+        }
     }
     tuned_[key] = best;
     return best;
@@ -685,7 +693,7 @@ float Context::runUntil(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
             const uint64_t end = std::min(j0->hPhaseEnd[ph], until);
             if (end <= at) continue;
             LaunchKnobs k = knobsIn;
-            k.threads = autotune(jobs, k, nullptr, static_cast<int>(j0->hPhaseP[ph]), end - at);
+            k.threads = autotune(jobs, k, &total, static_cast<int>(j0->hPhaseP[ph]), end - at);
             while (lowest(at) && at < end) {
                 k.sliceSweeps = sliceTo(at, end);
                 total += runSlice(jobs, k);
@@ -693,7 +701,7 @@ float Context::runUntil(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
         }
         return total;
     }
-    if (knobs.autotune && knobs.threads == 0) knobs.threads = autotune(jobs, knobs, nullptr);
+    if (knobs.autotune && knobs.threads == 0) knobs.threads = autotune(jobs, knobs, &total);
     if (knobs.hipGraph && knobs.sliceSweeps > 0 && until == ~uint64_t{0}) {
         // Every chain starts together here, so ceil(longest / slice) replays
         // of one captured slice finish them all.

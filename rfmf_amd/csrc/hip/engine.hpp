@@ -241,11 +241,12 @@ public:
     std::unique_ptr<GraphedSlices> captureSlices(const std::vector<ResidentJob*>& jobs, uint64_t sliceSweeps,
                                                  int gridCap, int threads);
     // Autotuner: times each thread-count candidate on one short slice of the
-    // live run (output-invariant), caches and returns the fastest.
-    // `keyP` (0: the jobs' largest P) and `budget` (0: the shortest remaining
-    // run) narrow the tuning to one phase: the cache key uses keyP and the
-    // trials together stay within a tenth of the budget.
-    int autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* trialMs, int keyP = 0,
+    // live run (output-invariant), caches and returns the fastest; the trial
+    // launches' kernel ms are added to *spentMs.  `keyP` (0: the jobs'
+    // largest P) and `budget` (0: the shortest remaining run) narrow the
+    // tuning to one phase: the cache key uses keyP and the trials together
+    // stay within a twenty-fifth of the budget.
+    int autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* spentMs, int keyP = 0,
                  uint64_t budget = 0);
     // Replays the captured slice `times` times; returns kernel-side ms.
     float replaySlices(GraphedSlices& g, int times);

@@ -223,13 +223,12 @@ int Context::chooseThreads(int bits, int mode, int64_t maxN, int64_t chains, int
     const auto it = tuned_.find(tuneKey(bits, mode, maxN, chains, maxP));
     if (it != tuned_.end()) return it->second;
     // Heuristic: with several chains per CU the 4-wave tile keeps every SIMD
-    // busy across chains; with fewer chains than CUs, more waves per chain
-    // cut the per-sweep latency instead.
+    // busy across chains; with about one chain per CU or fewer, 8 waves per
+    // chain cut the per-sweep latency (16 waves add more per-wave work than
+    // they hide).
     const double perCu = static_cast<double>(chains) / std::max(1, props_.multiProcessorCount);
-    int t = 256;
+    int t = 512;
     if (perCu >= 2.0) t = maxN <= 2048 ? 128 : 256;
-    else if (perCu >= 1.0) t = 512;
-    else t = maxN >= 4096 ? 1024 : 512;
     while (t > 128 && !fitsLds(maxN, bits, mode, maxP, t)) t /= 2;
     return t;
 }

@@ -207,15 +207,19 @@ typedef struct rfmf_tuning {
     int32_t autotune;           /* 1: time every thread count on one short slice of the live run
                                    first (exact: geometry never changes a result), cache the
                                    fastest per launch shape, run the rest with it */
-    int32_t reserved[2];
+    int32_t score_table;        /* 8-bit state words: 0 score from a 2048-entry LDS table when that
+                                   keeps as many chains resident, 1 always, -1 never */
+    int32_t reserved;
 } rfmf_tuning;
 
 /* Geometry of the context's last batch launch: threads per chain, grid,
- * state word bits, resident workgroups per CU, and the autotuner's trial
- * ms per sweep for 128/256/512/1024 threads (0 where not tried).  Replaces the
+ * state word bits, resident workgroups per CU, the autotuner's trial ms per
+ * sweep for 128/256/512/1024 threads (0 where not tried), and the score
+ * arithmetic (0 int32, 1 int32 with 24-bit multiplies, 2 int64, 3 24-bit with
+ * the LDS score table).  Any output pointer may be NULL.  Replaces the
  * reference's hard-coded V100 launch constants (src/tuning.hpp:8-18). */
 rfmf_status rfmf_ctx_last_launch(rfmf_ctx* ctx, int32_t* threads, int32_t* grid, int32_t* word_bits,
-                                 int32_t* occupancy, float* trial_ms_per_sweep);
+                                 int32_t* occupancy, float* trial_ms_per_sweep, int32_t* score_mode);
 
 /* Runs every job's chains to completion in persistent launches that pack all
  * jobs together; returns the kernel-only time of all launches. */

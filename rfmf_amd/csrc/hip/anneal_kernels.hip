@@ -338,8 +338,10 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                                                   int nItems, uint32_t* __restrict__ sched,
                                                   LdsPlan plan, uint64_t sliceSweeps) {
     constexpr int NW = NT / 64;
-    constexpr bool kMul24 = kMode == kScore24;
+    constexpr bool kTab = kMode == kScoreTab;
+    constexpr bool kMul24 = kMode == kScore24 || kTab;
     constexpr bool kWide = kMode == kScore64;
+    static_assert(!kTab || kBits == 8, "the score table needs 8-bit words (field < 128)");
     using Score = std::conditional_t<kWide, int64_t, int32_t>;
     constexpr Score kNoScore = kWide ? static_cast<Score>(INT64_MIN) : static_cast<Score>(INT32_MIN);
     using F = WordFmt<kBits>;
@@ -357,7 +359,17 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
     __shared__ __align__(16) uint32_t lutRaw[32];
     uint32_t* lutb = lutRaw;
     int64_t* lutw = reinterpret_cast<int64_t*>(lutRaw);
+    // kScoreTab: gtab[r] = LUT[r & 15] + (r >> 4 & 127) for the 11-bit field
+    // r, independent of beta.  A score is then beta * field + gtab[r], the
+    // true score plus bias * beta: the same shift for every vertex of a sweep,
+    // so the argmax is unchanged, and |beta * field| < 2^23 * 128 keeps it
+    // inside int32.  Filled once per launch, before the first barrier.
+    __shared__ int32_t gtab[kTab ? 2048 : 1];
+    if constexpr (kTab) {
+        for (int i = threadIdx.x; i < 2048; i += NT) gtab[i] = kGumbelCoarse[i & 15] + ((i >> 4) & 0x7F);
+    }
     auto setLut = [&](int32_t b) {
+        if constexpr (kTab) return;
         const int t = threadIdx.x;
         if (t < 16) {
             if constexpr (kWide)
@@ -503,6 +515,8 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                         Score score;
                         if constexpr (kWide)
                             score = scoreWide(beta, blk.field(j), lutw[r & 15u], fine);
+                        else if constexpr (kTab)
+                            score = __mul24(beta, static_cast<int32_t>(blk.field(j))) + gtab[r & 0x7FFu];
                         else
                             score = scoreOf<kMul24>(beta, blk.field(j), lutb[r & 15u], fine);
                         if constexpr (kLane) {
@@ -881,6 +895,9 @@ static const void* kernelForMode(int mode, int threads) {
             if constexpr (kBits <= 16) return kernelFor<kBits, dev::kScore24>(threads);
             return nullptr;
         case dev::kScore64: return kernelFor<kBits, dev::kScore64>(threads);
+        case dev::kScoreTab:   // 8-bit words only
+            if constexpr (kBits == 8) return kernelFor<kBits, dev::kScoreTab>(threads);
+            return nullptr;
         default: return nullptr;
     }
 }

@@ -16,17 +16,24 @@ namespace dev {
 constexpr int kThreadChoices[4] = {128, 256, 512, 1024};
 constexpr int kThreads = 256;
 inline bool validThreads(int nt) { return nt == 128 || nt == 256 || nt == 512 || nt == 1024; }
-// Static LDS of the slice kernel (the 16-entry biased Gumbel LUT, 32-bit
-// for the int32 score modes, 64-bit for the int64 one).
-constexpr uint32_t kStaticLds = 128;
+// Static LDS of the slice kernel: the 16-entry biased Gumbel LUT (32-bit for
+// the int32 score modes, 64-bit for the int64 one) plus 16 B of slack, and for
+// kScoreTab the 2048-entry score table.
+constexpr uint32_t kStaticLds = 144;
+constexpr uint32_t kScoreTabLds = 2048 * 4;
 
 // How a vertex's score = Gumbel + beta * spin*S is computed:
 //   kScore32: int32 arithmetic mod 2^32 (exact whenever the int32 gate holds);
 //   kScore24: the same with one v_mad_i32_i24 (|beta| < 2^23, 8/16-bit words);
 //   kScore64: exact int64 scores, for schedules the int32 gate refuses
-//             (the reference's two-pass path, src/anneal_kernels.cu:350-378).
-enum ScoreMode : int { kScore32 = 0, kScore24 = 1, kScore64 = 2 };
-constexpr int kScoreModes = 3;
+//             (the reference's two-pass path, src/anneal_kernels.cu:350-378);
+//   kScoreTab: kScore24 for 8-bit words with the Gumbel quantile and its
+//             7-bit fine offset read from one 2048-entry static table per
+//             workgroup (8 KB of LDS), so a score is one v_mad_i32_i24; chosen
+//             at launch when the table costs no resident chains.
+enum ScoreMode : int { kScore32 = 0, kScore24 = 1, kScore64 = 2, kScoreTab = 3 };
+constexpr int kScoreModes = 4;
+inline uint32_t staticLds(int mode) { return mode == kScoreTab ? kStaticLds + kScoreTabLds : kStaticLds; }
 
 // Byte offsets of the LDS regions of one workgroup (host-computed from the
 // largest graph and the largest P of the launch).

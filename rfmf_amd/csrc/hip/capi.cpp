@@ -353,7 +353,7 @@ rfmf_status rfmf_ctx_device_info(rfmf_ctx* ctx, int32_t* cu_count, int64_t* lds_
 }
 
 rfmf_status rfmf_ctx_last_launch(rfmf_ctx* ctx, int32_t* threads, int32_t* grid, int32_t* word_bits,
-                                 int32_t* occupancy, float* trial_ms_per_sweep) {
+                                 int32_t* occupancy, float* trial_ms_per_sweep, int32_t* score_mode) {
     return guarded([&] {
         if (!ctx) throw Error(Status::InvalidArg, "null context");
         std::lock_guard<std::mutex> lk(ctx->ctx->mutex());
@@ -362,6 +362,7 @@ rfmf_status rfmf_ctx_last_launch(rfmf_ctx* ctx, int32_t* threads, int32_t* grid,
         if (grid) *grid = r.grid;
         if (word_bits) *word_bits = r.bits;
         if (occupancy) *occupancy = r.occupancy;
+        if (score_mode) *score_mode = r.mode;
         if (trial_ms_per_sweep)
             for (int i = 0; i < 4; ++i) trial_ms_per_sweep[i] = r.trialMsPerSweep[i];
     });
@@ -434,6 +435,9 @@ rfmf::LaunchKnobs knobsFrom(const rfmf_tuning* t) {
         k.hipGraph = t->use_hip_graph != 0;
         k.threads = t->threads_per_chain;
         k.autotune = t->autotune != 0;
+        if (t->score_table < -1 || t->score_table > 1)
+            throw Error(Status::InvalidArg, "score_table must be -1, 0 or 1");
+        k.scoreTable = t->score_table;
     }
     return k;
 }
@@ -510,7 +514,7 @@ rfmf_status rfmf_run_advance(rfmf_run* run, int64_t sweeps, const rfmf_tuning* t
         if (active > 0) {
             if (k.hipGraph) {
                 if (!run->graph || run->graphSlice != sweeps || run->graphActive != active) {
-                    run->graph = c.captureSlices(run->ptrs, static_cast<uint64_t>(sweeps), k.gridCap, k.threads);
+                    run->graph = c.captureSlices(run->ptrs, static_cast<uint64_t>(sweeps), k);
                     run->graphSlice = sweeps;
                     run->graphActive = active;
                 }

@@ -216,7 +216,7 @@ uint64_t Context::tuneKey(int bits, int mode, int64_t maxN, int64_t chains, int 
 
 bool Context::fitsLds(int64_t maxN, int bits, int mode, int maxP, int threads) const {
     const dev::LdsPlan p = dev::makeLdsPlan(maxN, bits, maxP, threads, mode);
-    return static_cast<size_t>(p.total) + dev::kStaticLds <= props_.sharedMemPerBlock;
+    return static_cast<size_t>(p.total) + dev::staticLds(mode) <= props_.sharedMemPerBlock;
 }
 
 int Context::chooseThreads(int bits, int mode, int64_t maxN, int64_t chains, int maxP) const {
@@ -324,7 +324,7 @@ HostJob Context::prepareJob(const JobSpec& spec, const HostGraph& g) const {
     // The smallest tile (128 threads) has the smallest plan; launches pick a
     // tile that fits.
     const dev::LdsPlan plan = dev::makeLdsPlan(n, hj.bits, maxP, 128, hj.mode);
-    if (static_cast<size_t>(plan.total) + dev::kStaticLds > props_.sharedMemPerBlock)
+    if (static_cast<size_t>(plan.total) + dev::staticLds(hj.mode) > props_.sharedMemPerBlock)
         throw Error(Status::Unsupported, "chain state of n=" + std::to_string(n) + " (" +
                                              std::to_string(plan.total) +
                                              " B) exceeds the LDS per workgroup");
@@ -443,8 +443,10 @@ float Context::runSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
     return waitSlice(h);
 }
 
-std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<ResidentJob*>& jobs, int gridCap,
-                                                      int threads) {
+std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<ResidentJob*>& jobs,
+                                                      const LaunchKnobs& knobs) {
+    const int gridCap = knobs.gridCap;
+    const int threads = knobs.threads;
     // Group unfinished chains by kernel variant; items sorted by descending n
     // so the persistent workgroups take the longest chains first.
     constexpr int kGroups = 3 * dev::kScoreModes;
@@ -475,17 +477,36 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
         if (!dev::validThreads(g.threads))
             throw Error(Status::InvalidArg, "threads per chain must be 128, 256, 512 or 1024");
         g.plan = dev::makeLdsPlan(maxN[gi], g.bits, maxP[gi], g.threads, g.mode);
-        if (static_cast<size_t>(g.plan.total) + dev::kStaticLds > props_.sharedMemPerBlock)
+        if (static_cast<size_t>(g.plan.total) + dev::staticLds(g.mode) > props_.sharedMemPerBlock)
             throw Error(Status::Unsupported, "combined launch needs " + std::to_string(g.plan.total) +
                                                  " B of LDS per workgroup");
-        const int occ = std::max(1, occupancy(g.bits, g.mode, g.threads, g.plan.total));
-        int grid = occ * props_.multiProcessorCount;
-        if (gridCap > 0) grid = std::min(grid, gridCap);
-        g.grid = static_cast<int>(std::min<int64_t>(grid, static_cast<int64_t>(g.items.size())));
+        int occ = std::max(1, occupancy(g.bits, g.mode, g.threads, g.plan.total));
+        const int64_t nItems = static_cast<int64_t>(g.items.size());
+        auto residentOf = [&](int o) {
+            int64_t r = static_cast<int64_t>(o) * props_.multiProcessorCount;
+            if (gridCap > 0) r = std::min<int64_t>(r, gridCap);
+            return std::min(r, nItems);
+        };
+        // 8-bit words with 24-bit scores: the table variant (one mad per
+        // score, 8 KB more static LDS) replaces the per-sweep LUT when it
+        // keeps every chain of the launch resident that the LUT variant would.
+        if (g.bits == 8 && g.mode == dev::kScore24 && knobs.scoreTable >= 0) {
+            const dev::LdsPlan tp = dev::makeLdsPlan(maxN[gi], 8, maxP[gi], g.threads, dev::kScoreTab);
+            if (static_cast<size_t>(tp.total) + dev::staticLds(dev::kScoreTab) <= props_.sharedMemPerBlock) {
+                const int occTab = occupancy(8, dev::kScoreTab, g.threads, tp.total);
+                if (occTab > 0 && (knobs.scoreTable > 0 || residentOf(occTab) >= residentOf(occ))) {
+                    g.mode = dev::kScoreTab;
+                    g.plan = tp;
+                    occ = occTab;
+                }
+            }
+        }
+        g.grid = static_cast<int>(residentOf(occ));
         last_.threads = g.threads;
         last_.grid = g.grid;
         last_.bits = g.bits;
         last_.occupancy = occ;
+        last_.mode = g.mode;
         out.push_back(std::move(g));
     }
     return out;
@@ -494,7 +515,7 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
 Context::SliceHandle Context::launchSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs) {
     hipCheck(hipSetDevice(device_), "hipSetDevice");
     SliceHandle h;
-    std::vector<LaunchGroup> groups = planGroups(jobs, knobs.gridCap, knobs.threads);
+    std::vector<LaunchGroup> groups = planGroups(jobs, knobs);
     const uint64_t slice = knobs.sliceSweeps > 0 ? static_cast<uint64_t>(knobs.sliceSweeps) : ~uint64_t{0};
     size_t nItems = 0;
     for (const LaunchGroup& g : groups) nItems += g.items.size();
@@ -542,13 +563,13 @@ Context::GraphedSlices::~GraphedSlices() {
 }
 
 std::unique_ptr<Context::GraphedSlices> Context::captureSlices(const std::vector<ResidentJob*>& jobs,
-                                                               uint64_t sliceSweeps, int gridCap,
-                                                               int threads) {
+                                                               uint64_t sliceSweeps,
+                                                               const LaunchKnobs& knobs) {
     hipCheck(hipSetDevice(device_), "hipSetDevice");
     auto g = std::make_unique<GraphedSlices>();
     g->jobs = jobs;
     g->slice = sliceSweeps;
-    std::vector<LaunchGroup> groups = planGroups(jobs, gridCap, threads);
+    std::vector<LaunchGroup> groups = planGroups(jobs, knobs);
     // Item arrays go up before the capture; the graph holds kernel nodes only.
     for (LaunchGroup& lg : groups) {
         g->itemBufs.emplace_back(lg.items.size() * sizeof(dev::ChainItem));
@@ -708,7 +729,7 @@ float Context::runUntil(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
         for (const ResidentJob* j : jobs)
             for (uint64_t s : j->stepsDone) longest = std::max(longest, j->totalSteps - s);
         if (longest > 0) {
-            auto g = captureSlices(jobs, static_cast<uint64_t>(knobs.sliceSweeps), knobs.gridCap, knobs.threads);
+            auto g = captureSlices(jobs, static_cast<uint64_t>(knobs.sliceSweeps), knobs);
             const uint64_t reps = (longest + knobs.sliceSweeps - 1) / knobs.sliceSweeps;
             total += replaySlices(*g, static_cast<int>(reps));
         }

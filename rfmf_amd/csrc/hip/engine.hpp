@@ -160,6 +160,7 @@ struct LaunchKnobs {
     bool hipGraph = false;
     int threads = 0;           // threads per chain (128/256/512/1024); 0: tuned or heuristic
     bool autotune = false;     // time the candidates on the run's first slices
+    int scoreTable = 0;        // 8-bit words: 0 table when it costs no resident chains, 1 always, -1 never
 };
 
 // What the last launch of a context used (for reports).
@@ -168,6 +169,7 @@ struct LaunchReport {
     int grid = 0;
     int bits = 0;
     int occupancy = 0;
+    int mode = 0;   // dev::ScoreMode of the last group
     bool tuned = false;
     float trialMsPerSweep[4] = {0, 0, 0, 0};   // by kThreadChoices, 0 if not tried
 };
@@ -239,7 +241,7 @@ public:
         ~GraphedSlices();
     };
     std::unique_ptr<GraphedSlices> captureSlices(const std::vector<ResidentJob*>& jobs, uint64_t sliceSweeps,
-                                                 int gridCap, int threads);
+                                                 const LaunchKnobs& knobs);
     // Autotuner: times each thread-count candidate on one short slice of the
     // live run (output-invariant), caches and returns the fastest; the trial
     // launches' kernel ms are added to *spentMs.  `keyP` (0: the jobs'
@@ -271,7 +273,7 @@ private:
         dev::LdsPlan plan{};
         int grid = 1;
     };
-    std::vector<LaunchGroup> planGroups(const std::vector<ResidentJob*>& jobs, int gridCap, int threads);
+    std::vector<LaunchGroup> planGroups(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs);
     static uint64_t tuneKey(int bits, int mode, int64_t maxN, int64_t chains, int maxP);
     bool fitsLds(int64_t maxN, int bits, int mode, int maxP, int threads) const;
     std::unordered_map<uint64_t, int> tuned_;

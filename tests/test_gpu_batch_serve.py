@@ -100,6 +100,26 @@ def test_threads_per_chain_and_autotune_invariance(graphs, threads, autotune):
         same(j, one_shot(j), f"threads {threads} {k}")
 
 
+@pytest.mark.parametrize("score_table,mode", [(-1, "int32-mul24"), (0, "mul24-table"), (1, "mul24-table")])
+def test_score_table_choice_invariance(graphs, score_table, mode):
+    # 8-bit words with 24-bit scores run either with the per-sweep biased LUT
+    # or with the static LDS score table; the choice never changes a result.
+    print(f"[batch] score_table={score_table}", flush=True)
+    jobs = _mixed_jobs(graphs)
+    torus = make_job(graphs["torus"], [21, 22, 23], LADDER, [20] * 8)
+    with R.Context(0) as ctx:
+        ids = {k: ctx.upload_graph(g) for k, g in graphs.items()}
+        ctx.batch_run([(ids[k], j) for k, j in jobs], slice_sweeps=45, score_table=score_table)
+        ctx.batch_run([(ids["torus"], torus)], score_table=score_table)
+        assert ctx.last_launch()["score_mode"] == mode
+    for k, j in jobs:
+        same(j, one_shot(j), f"score_table {score_table} {k}")
+    end, best, cv, bv = run_oracle_chains(torus.csr, torus.init, torus.seeds, torus.beta, torus.phases,
+                                          torus.iters)
+    assert np.array_equal(torus.end_spins, end) and np.array_equal(torus.best_spins, best)
+    assert np.array_equal(torus.best_vals, bv)
+
+
 def test_chain_isolation_beyond_resident_capacity(gset):
     # 3000 chains of an 800-vertex graph exceed the resident workgroups of
     # 256 CUs, so the persistent workgroups take several chains each.

@@ -307,6 +307,20 @@ __device__ __forceinline__ void foldLanes(S& s, int32_t& v, int lane) {
     }
 }
 
+// Sum over the 64 lanes, left in every lane: rotations inside each 16-lane
+// row, then the row and half swaps (all on the VALU).
+__device__ __forceinline__ int32_t waveSum32(int32_t x) {
+    x += rowRor<8>(x);
+    x += rowRor<4>(x);
+    x += rowRor<2>(x);
+    x += rowRor<1>(x);
+    int32_t a, b;
+    swapPair<16>(x, a, b);
+    x = a + b;
+    swapPair<32>(x, a, b);
+    return a + b;
+}
+
 __device__ __forceinline__ int64_t waveSum64(int64_t x) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
@@ -542,8 +556,10 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
 
                 if constexpr (kLane) {
                     if (wave >= busyWaves) return;
+                    // Block index -> vertex; an empty slot's -1 becomes j - 8,
+                    // still negative, so it stays empty.
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) accV[j] = accV[j] >= 0 ? 8 * accV[j] + j : -1;
+                    for (int j = 0; j < 8; ++j) accV[j] = 8 * accV[j] + j;
                     if (P >= 8) {
                         // Lanes l and l' share groups iff l = l' (mod Q); slot j
                         // of lane l feeds group 8 (l mod Q) + j.  Reduce-scatter
@@ -698,6 +714,9 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                 const int passes = (P + perPass - 1) / perPass;
                 const int sub = tid & (L - 1);
                 for (int pass = 0; pass < passes; ++pass) {
+                    // Groups rise with the pass and the lane: a wave whose
+                    // first group is past P has nothing left to do.
+                    if (pass * perPass + ((wave << 6) >> lgL) >= P) break;
                     const int g = pass * perPass + (tid >> lgL);
                     int32_t b = -1, sb = 0;
                     if (g < P) { b = pk[g]; sb = ps[g]; }
@@ -740,9 +759,13 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                             if (two) apply(e1.x, 2 * e1.y, gx1, pk1, wd1);
                         }
                     }
-#pragma unroll
-                    for (int m = 1; m < 64; m <<= 1)
-                        if (m < L) own += __shfl_xor(own, m, 64);
+                    // Sum over the winner's L lanes (an aligned lane group).
+                    if (L > 1) own += laneXor<1>(own, lane);
+                    if (L > 2) own += laneXor<2>(own, lane);
+                    if (L > 4) own += laneXor<4>(own, lane);
+                    if (L > 8) own += laneXor<8>(own, lane);
+                    if (L > 16) { int32_t a, b; swapPair<16>(own, a, b); own = a + b; }
+                    if (L > 32) { int32_t a, b; swapPair<32>(own, a, b); own = a + b; }
                     if (b >= 0 && sub == 0) {
                         dsum += pd[g];
                         // delta(b) -> -delta(b) + own, spin bit toggled: one add.
@@ -751,7 +774,11 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                     }
                 }
             }
-            dsum = waveSum64(dsum);
+            // |cut delta| <= 3 * B * P < 2^31 below 32-bit words.
+            if constexpr (kBits < 32)
+                dsum = waveSum32(static_cast<int32_t>(dsum));
+            else
+                dsum = waveSum64(dsum);
             if (lane == 0) dpart[wave] = dsum;
             __syncthreads();   // B3: spins, fields, LUT and partial sums complete
 

@@ -472,8 +472,21 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
         // fewer blocks than threads).  The others skip the per-group
         // reduction: their partial entries stay zero (empty).
         const int busyWaves = nBlk >= NT ? NW : (nBlk + 63) >> 6;
+        // Best-spin snapshot: the spins after an improving sweep, taken before
+        // the next sweep's flips (and at the end of the slice).  Each thread
+        // packs its own blocks; nothing else touches bl8 until the copy-out.
+        auto snapshot = [&]() {
+            for (int b = tid; b < nBlk; b += NT) {
+                Block8<kBits> blk;
+                blk.load(wl, b);
+                uint32_t m = blk.spinByte();
+                if (8 * b + 8 > n) m &= (1u << (n - 8 * b)) - 1u;
+                bl8[b] = static_cast<uint8_t>(m);
+            }
+        };
 
         while (step < stepEnd) {
+            if (improved) snapshot();
             if (step >= phaseStop && ph < nPhases - 1) {
                 while (ph < nPhases - 1 && uni(phaseEnd[ph]) <= step) ++ph;
                 phaseStop = uni(phaseEnd[ph]);
@@ -516,7 +529,6 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                 auto scoreBlock = [&](int b, bool tail) {
                     Block8<kBits> blk;
                     blk.load(wl, b);
-                    if (improved && !idPass) bl8[b] = static_cast<uint8_t>(blk.spinByte());
                     uint32_t c0 = sLo, c1 = sHi, c2 = static_cast<uint32_t>(b), c3 = 0u;
                     philox10(c0, c1, c2, c3, k0, k1);
                     const uint32_t o[4] = {c0, c1, c2, c3};
@@ -791,17 +803,8 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
             ++step;
         }
 
-        // The snapshot rides one sweep late in SELECT; an improving final
-        // sweep is captured here.
-        if (improved) {
-            for (int b = tid; b < nBlk; b += NT) {
-                Block8<kBits> blk;
-                blk.load(wl, b);
-                uint32_t m = blk.spinByte();
-                if (8 * b + 8 > n) m &= (1u << (n - 8 * b)) - 1u;
-                bl8[b] = static_cast<uint8_t>(m);
-            }
-        }
+        // An improving last sweep of the slice.
+        if (improved) snapshot();
         __syncthreads();
         {
             uint4* gw = reinterpret_cast<uint4*>(it.words);

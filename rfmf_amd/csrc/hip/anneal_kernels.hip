@@ -61,24 +61,65 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
-// Philox4x32-10 (Salmon et al. 2011): 10 rounds, key bumped between rounds.
-// The step words and key are wave-uniform, so the compiler keeps the first
-// round's uniform products on the scalar unit.
-__device__ __forceinline__ void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2,
-                                         uint32_t& c3, uint32_t k0, uint32_t k1) {
+// Philox4x32-10 (Salmon et al. 2011) of the counter (sLo, sHi, b, 0) under
+// the key (k0, k1): 10 rounds of
+//   (c0, c1, c2, c3) <- (hi(M1 c2) ^ c1 ^ k0, lo(M1 c2), hi(M0 c0) ^ c3 ^ k1, lo(M0 c0)),
+// the key bumped by (W0, W1) between rounds.  Only c2 = b differs between
+// the blocks of a sweep, so the products and xors that see no b are done
+// once per sweep on the scalar unit (PhiloxSweep): round 1 needs one vector
+// product, round 2 one, rounds 3-10 two each.
+struct PhiloxSweep {
+    uint32_t x1;     // sHi ^ k0: round 1 c0 = hi(M1 b) ^ x1
+    uint32_t u2a;    // round 2 c0 = lo(M1 b) ^ u2a
+    uint32_t u2b;    // round 2 c2 = hi(M0 c0) ^ u2b
+    uint32_t u3a;    // round 3 c0 = hi(M1 c2) ^ u3a
+    uint32_t k0, k1; // the key
+};
+__device__ __forceinline__ PhiloxSweep philoxSweep(uint32_t sLo, uint32_t sHi, uint32_t k0, uint32_t k1) {
+    PhiloxSweep p;
+    const uint64_t q0 = static_cast<uint64_t>(kPhiloxM0) * sLo;      // round 1, c0 side
+    const uint32_t c2r1 = static_cast<uint32_t>(q0 >> 32) ^ k1;        // round 1 c2 (c3 = 0)
+    const uint32_t c3r1 = static_cast<uint32_t>(q0);                   // round 1 c3
+    const uint64_t q1 = static_cast<uint64_t>(kPhiloxM1) * c2r1;      // round 2, c2 side
+    p.x1 = sHi ^ k0;
+    p.u2a = static_cast<uint32_t>(q1 >> 32) ^ (k0 + kPhiloxW0);
+    p.u2b = c3r1 ^ (k1 + kPhiloxW1);
+    p.u3a = static_cast<uint32_t>(q1) ^ (k0 + 2u * kPhiloxW0);        // round 2 c1 = lo(q1)
+    p.k0 = k0;
+    p.k1 = k1;
+    return p;
+}
+__device__ __forceinline__ void philoxBlock(const PhiloxSweep& s, uint32_t b, uint32_t (&o)[4]) {
+    const uint64_t p1 = static_cast<uint64_t>(kPhiloxM1) * b;         // round 1
+    uint32_t c0 = static_cast<uint32_t>(p1 >> 32) ^ s.x1;
+    uint32_t c1 = static_cast<uint32_t>(p1);
+    const uint64_t p0 = static_cast<uint64_t>(kPhiloxM0) * c0;        // round 2
+    c0 = c1 ^ s.u2a;
+    uint32_t c2 = static_cast<uint32_t>(p0 >> 32) ^ s.u2b;
+    uint32_t c3 = static_cast<uint32_t>(p0);
+    {                                                                  // round 3
+        const uint64_t a = static_cast<uint64_t>(kPhiloxM0) * c0;
+        const uint64_t d = static_cast<uint64_t>(kPhiloxM1) * c2;
+        c0 = static_cast<uint32_t>(d >> 32) ^ s.u3a;
+        c2 = xor3(static_cast<uint32_t>(a >> 32), c3, s.k1 + 2u * kPhiloxW1);
+        c1 = static_cast<uint32_t>(d);
+        c3 = static_cast<uint32_t>(a);
+    }
 #pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint64_t p0 = static_cast<uint64_t>(kPhiloxM0) * c0;
-        const uint64_t p1 = static_cast<uint64_t>(kPhiloxM1) * c2;
-        const uint32_t n0 = xor3(static_cast<uint32_t>(p1 >> 32), c1, k0);
-        const uint32_t n2 = xor3(static_cast<uint32_t>(p0 >> 32), c3, k1);
-        c1 = static_cast<uint32_t>(p1);
-        c3 = static_cast<uint32_t>(p0);
+    for (uint32_t r = 3; r < 10; ++r) {
+        const uint64_t a = static_cast<uint64_t>(kPhiloxM0) * c0;
+        const uint64_t d = static_cast<uint64_t>(kPhiloxM1) * c2;
+        const uint32_t n0 = xor3(static_cast<uint32_t>(d >> 32), c1, s.k0 + r * kPhiloxW0);
+        const uint32_t n2 = xor3(static_cast<uint32_t>(a >> 32), c3, s.k1 + r * kPhiloxW1);
+        c1 = static_cast<uint32_t>(d);
+        c3 = static_cast<uint32_t>(a);
         c0 = n0;
         c2 = n2;
-        k0 += kPhiloxW0;
-        k1 += kPhiloxW1;
     }
+    o[0] = c0;
+    o[1] = c1;
+    o[2] = c2;
+    o[3] = c3;
 }
 
 // 8 consecutive words of a vertex block, as loaded from LDS.
@@ -347,10 +388,18 @@ __device__ __forceinline__ int64_t scoreWide(int32_t beta, uint32_t field, int64
     return static_cast<int64_t>(beta) * static_cast<int64_t>(field) + lutBiased + static_cast<int64_t>(fine);
 }
 
+// Occupancy floor per variant: 6 waves per SIMD (at most 80 VGPRs) keeps 12
+// two-wave chains resident per CU in large batches; 32-bit words take 5 and
+// int64 scores 4, where fewer registers would spill.  The score-table
+// variant runs launches whose chains are all resident anyway, so its
+// registers go to the schedule (floor 1).
+template <int kBits, int kMode>
+constexpr int kSliceWavesPerEu = kMode == kScore64 ? 4 : kMode == kScoreTab ? 1 : (kBits == 32 ? 5 : 6);
+
 template <int kBits, int kMode, int NT>
-__global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ items,
-                                                  int nItems, uint32_t* __restrict__ sched,
-                                                  LdsPlan plan, uint64_t sliceSweeps) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kSliceWavesPerEu<kBits, kMode>)))
+void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __restrict__ sched, LdsPlan plan,
+                 uint64_t sliceSweeps) {
     constexpr int NW = NT / 64;
     constexpr bool kTab = kMode == kScoreTab;
     constexpr bool kMul24 = kMode == kScore24 || kTab;
@@ -503,6 +552,7 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
             const uint64_t s1 = step + 1;   // 1-based sweep id, the Philox counter
             const uint32_t sLo = static_cast<uint32_t>(s1);
             const uint32_t sHi = static_cast<uint32_t>(s1 >> 32);
+            const PhiloxSweep phx = philoxSweep(sLo, sHi, k0, k1);
             const bool pow2 = (P & (P - 1)) == 0;
             const bool laneReduce = pow2 && P <= kMaxLaneReduceP && ((8 * NT) % P) == 0;
 
@@ -529,9 +579,8 @@ __global__ __launch_bounds__(NT) void annealSlice(const ChainItem* __restrict__ 
                 auto scoreBlock = [&](int b, bool tail) {
                     Block8<kBits> blk;
                     blk.load(wl, b);
-                    uint32_t c0 = sLo, c1 = sHi, c2 = static_cast<uint32_t>(b), c3 = 0u;
-                    philox10(c0, c1, c2, c3, k0, k1);
-                    const uint32_t o[4] = {c0, c1, c2, c3};
+                    uint32_t o[4];
+                    philoxBlock(phx, static_cast<uint32_t>(b), o);
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {
                         const int v = 8 * b + j;

@@ -10,20 +10,24 @@
 // flips would (the winner-winner coupling corr[]), then updates the fields.
 //
 // The execution is MI355X-native and differs from the reference throughout:
-//   * one 256-thread workgroup (4 wave64) anneals one chain at a time, and the
-//     chain's whole state lives in LDS for the length of a slice: one packed
-//     word per vertex holding the spin bit and delta = spin*S (biased to stay
-//     non-negative; 8/16/32 bits chosen from the graph's max weighted degree)
-//     plus a best-spin bitmask;
-//   * scoring is one v_mad_i32_i24 per vertex: the bias is folded into a
-//     per-sweep copy of the Gumbel LUT (LUT - bias*beta, wrapping mod 2^32;
-//     the final score is exact because it fits int32 under the gate); when
+//   * one workgroup of 128-1024 threads (2-16 wave64) anneals one chain at a
+//     time, and the chain's whole state lives in LDS for the length of a
+//     slice: one packed word per vertex holding the spin bit and delta =
+//     spin*S (biased to stay non-negative; 8/16/32 bits chosen from the
+//     graph's max weighted degree) plus a best-spin bitmask;
+//   * scoring is one v_mad_i32_i24 per vertex: the bias is folded either into
+//     a per-sweep copy of the Gumbel LUT (LUT - bias*beta, wrapping mod 2^32;
+//     the final score is exact because it fits int32 under the gate) or, for
+//     8-bit words, into a static 2048-entry table of Gumbel values (the
+//     scores all shift by bias*beta, which leaves the argmax unchanged); when
 //     the gate fails, the same kernel scores in exact int64 and resolves
 //     (score, id) pairs instead of packed 64-bit keys;
+//   * Philox's block-independent half of rounds 1-3 is computed once per
+//     sweep on the scalar unit;
 //   * the per-group argmax is kept in registers (each lane's 8 vertex slots
-//     map to fixed groups when P divides 8*threads), folded across lanes with
-//     a butterfly and across the 4 waves through LDS: no atomics on the hot
-//     path;
+//     map to fixed groups when P divides 8*threads), combined across lanes by
+//     a reduce-scatter on DPP and v_permlane{16,32}_swap, and across waves
+//     through LDS: no atomics on the hot path;
 //   * the winner map for corr[] is implicit: vertex x can only win group
 //     x mod P, so "x is a winner" is pick[x mod P] == x and no n-sized
 //     winnerGroup array exists;

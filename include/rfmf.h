@@ -207,8 +207,9 @@ typedef struct rfmf_tuning {
     int32_t autotune;           /* 1: time every thread count on one short slice of the live run
                                    first (exact: geometry never changes a result), cache the
                                    fastest per launch shape, run the rest with it */
-    int32_t score_table;        /* 8-bit state words: 0 score from a 2048-entry LDS table when that
-                                   keeps as many chains resident, 1 always, -1 never */
+    int32_t score_table;        /* 8-bit state words with 24-bit scores: 0 score from a 2048-entry
+                                   LDS Gumbel table when that keeps as many chains resident, 1
+                                   whenever the table fits in LDS, -1 never; output-invariant */
     int32_t reserved;
 } rfmf_tuning;
 

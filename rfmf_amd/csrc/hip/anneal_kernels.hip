@@ -126,20 +126,61 @@ __device__ __forceinline__ void philoxBlock(const PhiloxSweep& s, uint32_t b, ui
     o[3] = c3;
 }
 
-// 8 consecutive words of a vertex block, as loaded from LDS.
+// Global-memory types: pointers in address space 1 compile to global_*
+// instructions with a scalar base (generic pointers would give flat_*, which
+// also count against the LDS wait counter).  Native vector types, so that a
+// dereference is one load (HIP's uint2/int2 are classes whose copy
+// constructors would take the address back to the generic space).
+template <class T>
+using GlobalOf = __attribute__((address_space(1))) T;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t i32x2 __attribute__((ext_vector_type(2)));
+
+template <class T>
+__device__ __forceinline__ const GlobalOf<T>* globalIn(const T* p) {
+    return (const GlobalOf<T>*)p;
+}
+template <class T>
+__device__ __forceinline__ GlobalOf<T>* globalIn(T* p) {
+    return (GlobalOf<T>*)p;
+}
+
+// Chain-state accesses through a generic pointer that addresses HBM
+// (kGlobal) or LDS.
+template <bool kGlobal, class T>
+__device__ __forceinline__ T loadAt(const T* p) {
+    if constexpr (kGlobal) return *globalIn(p);
+    else return *p;
+}
+template <bool kGlobal, class T>
+__device__ __forceinline__ void storeAt(T* p, T v) {
+    if constexpr (kGlobal) *globalIn(p) = v;
+    else *p = v;
+}
+template <bool kGlobal>
+__device__ __forceinline__ void atomicAddAt(uint32_t* p, uint32_t v) {
+    if constexpr (kGlobal)
+        __hip_atomic_fetch_add(globalIn(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        atomicAdd(p, v);
+}
+
+// 8 consecutive words of a vertex block, as loaded from LDS (or HBM).
 template <int kBits> struct Block8 {
     static constexpr int kDw = kBits / 4;   // dwords holding 8 words
     uint32_t d[kDw];
+    template <bool kGlobal = false>
     __device__ __forceinline__ void load(const uint32_t* wl, int b) {
         if constexpr (kBits == 8) {
-            const uint2 q = reinterpret_cast<const uint2*>(wl)[b];
+            const u32x2 q = loadAt<kGlobal>(reinterpret_cast<const u32x2*>(wl) + b);
             d[0] = q.x; d[1] = q.y;
         } else if constexpr (kBits == 16) {
-            const uint4 q = reinterpret_cast<const uint4*>(wl)[b];
+            const u32x4 q = loadAt<kGlobal>(reinterpret_cast<const u32x4*>(wl) + b);
             d[0] = q.x; d[1] = q.y; d[2] = q.z; d[3] = q.w;
         } else {
-            const uint4 q0 = reinterpret_cast<const uint4*>(wl)[2 * b];
-            const uint4 q1 = reinterpret_cast<const uint4*>(wl)[2 * b + 1];
+            const u32x4 q0 = loadAt<kGlobal>(reinterpret_cast<const u32x4*>(wl) + 2 * b);
+            const u32x4 q1 = loadAt<kGlobal>(reinterpret_cast<const u32x4*>(wl) + 2 * b + 1);
             d[0] = q0.x; d[1] = q0.y; d[2] = q0.z; d[3] = q0.w;
             d[4] = q1.x; d[5] = q1.y; d[6] = q1.z; d[7] = q1.w;
         }
@@ -170,23 +211,23 @@ template <int kBits> struct Block8 {
     }
 };
 
-template <int kBits>
+template <int kBits, bool kGlobal = false>
 __device__ __forceinline__ uint32_t ldsWord(const uint32_t* wl, int x) {
     if constexpr (kBits == 32) {
-        return wl[x];
+        return loadAt<kGlobal>(wl + x);
     } else {
         constexpr int per = 32 / kBits;
-        return (wl[x / per] >> (kBits * (x % per))) & ((1u << kBits) - 1u);
+        return (loadAt<kGlobal>(wl + x / per) >> (kBits * (x % per))) & ((1u << kBits) - 1u);
     }
 }
 
 // Adds d (mod 2^kBits, applied without carry or borrow because the word's
 // old and new values are both valid words) to word x.
-template <int kBits>
+template <int kBits, bool kGlobal = false>
 __device__ __forceinline__ void ldsAddWord(uint32_t* wl, int x, uint32_t d) {
     constexpr int per = 32 / kBits;
     const int sh = kBits * (x % per);
-    atomicAdd(&wl[x / per], d << sh);
+    atomicAddAt<kGlobal>(&wl[x / per], d << sh);
 }
 
 __device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -393,15 +434,20 @@ __device__ __forceinline__ int64_t scoreWide(int32_t beta, uint32_t field, int64
 }
 
 // Occupancy floor per variant: 6 waves per SIMD (at most 80 VGPRs) keeps 12
-// two-wave chains resident per CU in large batches; 32-bit words take 5 and
-// int64 scores 4, where fewer registers would spill.  The score-table
-// variant runs launches whose chains are all resident anyway, so its
-// registers go to the schedule (floor 1).
-template <int kBits, int kMode>
-constexpr int kSliceWavesPerEu = kMode == kScore64 ? 4 : kMode == kScoreTab ? 1 : (kBits == 32 ? 5 : 6);
+// two-wave chains resident per CU in large batches; 32-bit words and
+// HBM-resident state take 5 and int64 scores 4, where fewer registers would
+// spill.  The score-table variant runs launches whose chains are all
+// resident anyway, so its registers go to the schedule (floor 1).
+template <int kBits, int kMode, bool kHbm>
+constexpr int kSliceWavesPerEu =
+    kMode == kScore64 ? 4 : kMode == kScoreTab ? 1 : (kBits == 32 || kHbm) ? 5 : 6;
 
-template <int kBits, int kMode, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kSliceWavesPerEu<kBits, kMode>)))
+// kHbm: the chain's words and best bits stay in HBM and are updated in place
+// (graphs whose state does not fit LDS).  FLIP's global atomics complete in
+// L2, so after the sweep's last barrier each wave invalidates its L1 (an
+// agent-scope acquire) before the next sweep reads the words.
+template <int kBits, int kMode, int NT, bool kHbm>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kSliceWavesPerEu<kBits, kMode, kHbm>)))
 void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __restrict__ sched, LdsPlan plan,
                  uint64_t sliceSweeps) {
     constexpr int NW = NT / 64;
@@ -412,10 +458,10 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     using Score = std::conditional_t<kWide, int64_t, int32_t>;
     constexpr Score kNoScore = kWide ? static_cast<Score>(INT64_MIN) : static_cast<Score>(INT32_MIN);
     using F = WordFmt<kBits>;
+    static_assert(!kHbm || !kTab, "HBM-resident state runs the LUT variants");
     extern __shared__ __align__(16) unsigned char lds[];
-    uint32_t* wl = reinterpret_cast<uint32_t*>(lds + plan.wordsOff);
-    uint32_t* bl = reinterpret_cast<uint32_t*>(lds + plan.bestOff);
-    uint8_t* bl8 = reinterpret_cast<uint8_t*>(lds + plan.bestOff);
+    uint32_t* const ldsWords = reinterpret_cast<uint32_t*>(lds + plan.wordsOff);
+    uint32_t* const ldsBest = reinterpret_cast<uint32_t*>(lds + plan.bestOff);
     uint64_t* part = reinterpret_cast<uint64_t*>(lds + plan.partOff);
     uint32_t* partId = reinterpret_cast<uint32_t*>(lds + plan.partIdOff);   // int64 scores only
     int32_t* pk = reinterpret_cast<int32_t*>(lds + plan.pickOff);
@@ -465,29 +511,37 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
         // pointers, which the compiler would otherwise keep in VGPRs).
         const ChainItem& it = items[itemIdx];
         const int n = uni(it.n);
-        const uint32_t* __restrict__ offsets = it.offsets;
-        const int2* __restrict__ edges = reinterpret_cast<const int2*>(it.edges);
-        ChainState* st = it.st;
-        const int64_t* betaEnd = it.betaEnd;
-        const int32_t* betaVal = it.betaVal;
-        const uint64_t* phaseEnd = it.phaseEnd;
-        const uint32_t* phaseP = it.phaseP;
+        const GlobalOf<uint32_t>* __restrict__ offsets = globalIn(it.offsets);
+        const GlobalOf<i32x2>* __restrict__ edges = globalIn(reinterpret_cast<const i32x2*>(it.edges));
+        GlobalOf<ChainState>* st = globalIn(it.st);
+        const GlobalOf<int64_t>* betaEnd = globalIn(it.betaEnd);
+        const GlobalOf<int32_t>* betaVal = globalIn(it.betaVal);
+        const GlobalOf<uint64_t>* phaseEnd = globalIn(it.phaseEnd);
+        const GlobalOf<uint32_t>* phaseP = globalIn(it.phaseP);
         const int nRuns = uni(it.nRuns);
         const int nPhases = uni(it.nPhases);
         const uint32_t k0 = uni(static_cast<uint32_t>(it.key));
         const uint32_t k1 = uni(static_cast<uint32_t>(it.key >> 32));
 
-        // ---- load the chain's state into LDS ----
+        // ---- the chain's state: copied into LDS, or used in place ----
         const int wq = static_cast<int>(wordBytes(n, kBits) / 16);
         const int bq = static_cast<int>(bestBytes(n) / 16);
-        {
-            const uint4* gw = reinterpret_cast<const uint4*>(it.words);
-            uint4* lw = reinterpret_cast<uint4*>(wl);
+        uint32_t* wl;
+        uint32_t* bl;
+        if constexpr (kHbm) {
+            wl = it.words;
+            bl = it.bestbits;
+        } else {
+            wl = ldsWords;
+            bl = ldsBest;
+            const GlobalOf<u32x4>* gw = globalIn(reinterpret_cast<const u32x4*>(it.words));
+            u32x4* lw = reinterpret_cast<u32x4*>(wl);
             for (int i = tid; i < wq; i += NT) lw[i] = gw[i];
-            const uint4* gb = reinterpret_cast<const uint4*>(it.bestbits);
-            uint4* lb = reinterpret_cast<uint4*>(bl);
+            const GlobalOf<u32x4>* gb = globalIn(reinterpret_cast<const u32x4*>(it.bestbits));
+            u32x4* lb = reinterpret_cast<u32x4*>(bl);
             for (int i = tid; i < bq; i += NT) lb[i] = gb[i];
         }
+        uint8_t* const bl8 = reinterpret_cast<uint8_t*>(bl);
         int64_t cur = uni(st->cur);
         int64_t best = uni(st->best);
         uint64_t step = uni(st->step);
@@ -531,10 +585,10 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
         auto snapshot = [&]() {
             for (int b = tid; b < nBlk; b += NT) {
                 Block8<kBits> blk;
-                blk.load(wl, b);
+                blk.template load<kHbm>(wl, b);
                 uint32_t m = blk.spinByte();
                 if (8 * b + 8 > n) m &= (1u << (n - 8 * b)) - 1u;
-                bl8[b] = static_cast<uint8_t>(m);
+                storeAt<kHbm>(bl8 + b, static_cast<uint8_t>(m));
             }
         };
 
@@ -582,7 +636,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 for (int j = 0; j < 8; ++j) { accS[j] = kNoScore; accV[j] = -1; }
                 auto scoreBlock = [&](int b, bool tail) {
                     Block8<kBits> blk;
-                    blk.load(wl, b);
+                    blk.template load<kHbm>(wl, b);
                     uint32_t o[4];
                     philoxBlock(phx, static_cast<uint32_t>(b), o);
 #pragma unroll
@@ -742,7 +796,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 int32_t pick = -1, sp = 0, dl = 0;
                 if (key != 0ull) {
                     pick = static_cast<int32_t>(~(kWide ? kid : static_cast<uint32_t>(key)));
-                    const uint32_t w = ldsWord<kBits>(wl, pick);
+                    const uint32_t w = ldsWord<kBits, kHbm>(wl, pick);
                     sp = (w & F::kSpin) ? 1 : -1;
                     dl = static_cast<int32_t>(w & F::kField) - F::kBias;
                 }
@@ -798,14 +852,14 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                         for (uint32_t i0 = lo + sub; i0 < hi; i0 += 2 * L) {
                             const uint32_t i1 = i0 + static_cast<uint32_t>(L);
                             const bool two = i1 < hi;
-                            const int2 e0 = edges[i0];
-                            const int2 e1 = two ? edges[i1] : make_int2(e0.x, 0);
+                            const i32x2 e0 = edges[i0];
+                            const i32x2 e1 = two ? edges[i1] : i32x2{e0.x, 0};
                             const int gx0 = pow2 ? (e0.x & (P - 1)) : (e0.x % P);
                             const int gx1 = pow2 ? (e1.x & (P - 1)) : (e1.x % P);
                             const int32_t pk0 = pk[gx0];
                             const int32_t pk1 = pk[gx1];
-                            const uint32_t wd0 = ldsWord<kBits>(wl, e0.x);
-                            const uint32_t wd1 = ldsWord<kBits>(wl, e1.x);
+                            const uint32_t wd0 = ldsWord<kBits, kHbm>(wl, e0.x);
+                            const uint32_t wd1 = ldsWord<kBits, kHbm>(wl, e1.x);
                             auto apply = [&](int x, int32_t w2, int gx, int32_t pkx, uint32_t wd) {
                                 if (pkx == x) {
                                     // x flips too: its coupling enters b's own
@@ -817,7 +871,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                                 } else {
                                     // delta(x) += s_x * (-2 w s_b); x's spin is stable.
                                     const int32_t d = (wd & F::kSpin) ? -w2 * sb : w2 * sb;
-                                    ldsAddWord<kBits>(wl, x, static_cast<uint32_t>(d));
+                                    ldsAddWord<kBits, kHbm>(wl, x, static_cast<uint32_t>(d));
                                 }
                             };
                             apply(e0.x, 2 * e0.y, gx0, pk0, wd0);
@@ -835,7 +889,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                         dsum += pd[g];
                         // delta(b) -> -delta(b) + own, spin bit toggled: one add.
                         const uint32_t toggle = sb > 0 ? 0u - F::kSpin : F::kSpin;
-                        ldsAddWord<kBits>(wl, b, static_cast<uint32_t>(own) - 2u * static_cast<uint32_t>(pd[g]) + toggle);
+                        ldsAddWord<kBits, kHbm>(wl, b, static_cast<uint32_t>(own) - 2u * static_cast<uint32_t>(pd[g]) + toggle);
                     }
                 }
             }
@@ -846,6 +900,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 dsum = waveSum64(dsum);
             if (lane == 0) dpart[wave] = dsum;
             __syncthreads();   // B3: spins, fields, LUT and partial sums complete
+            if constexpr (kHbm) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
             int64_t total = 0;
 #pragma unroll
@@ -859,12 +914,12 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
         // An improving last sweep of the slice.
         if (improved) snapshot();
         __syncthreads();
-        {
-            uint4* gw = reinterpret_cast<uint4*>(it.words);
-            const uint4* lw = reinterpret_cast<const uint4*>(wl);
+        if constexpr (!kHbm) {
+            GlobalOf<u32x4>* gw = globalIn(reinterpret_cast<u32x4*>(it.words));
+            const u32x4* lw = reinterpret_cast<const u32x4*>(wl);
             for (int i = tid; i < wq; i += NT) gw[i] = lw[i];
-            uint4* gb = reinterpret_cast<uint4*>(it.bestbits);
-            const uint4* lb = reinterpret_cast<const uint4*>(bl);
+            GlobalOf<u32x4>* gb = globalIn(reinterpret_cast<u32x4*>(it.bestbits));
+            const u32x4* lb = reinterpret_cast<const u32x4*>(bl);
             for (int i = tid; i < bq; i += NT) gb[i] = lb[i];
         }
         if (tid == 0) {
@@ -958,45 +1013,46 @@ hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s) {
     return hipGetLastError();
 }
 
-// The compiled variants: word width x score mode x threads per chain.
-template <int kBits, int kMode>
+// The compiled variants: word width x score mode x threads per chain x
+// state in LDS or HBM.
+template <int kBits, int kMode, bool kHbm>
 static const void* kernelFor(int threads) {
     switch (threads) {
-        case 128: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 128>);
-        case 256: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 256>);
-        case 512: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 512>);
-        case 1024: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 1024>);
+        case 128: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 128, kHbm>);
+        case 256: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 256, kHbm>);
+        case 512: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 512, kHbm>);
+        case 1024: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 1024, kHbm>);
         default: return nullptr;
     }
 }
 
-template <int kBits>
+template <int kBits, bool kHbm>
 static const void* kernelForMode(int mode, int threads) {
     switch (mode) {
-        case dev::kScore32: return kernelFor<kBits, dev::kScore32>(threads);
+        case dev::kScore32: return kernelFor<kBits, dev::kScore32, kHbm>(threads);
         case dev::kScore24:   // 24-bit fields only (8/16-bit words)
-            if constexpr (kBits <= 16) return kernelFor<kBits, dev::kScore24>(threads);
+            if constexpr (kBits <= 16) return kernelFor<kBits, dev::kScore24, kHbm>(threads);
             return nullptr;
-        case dev::kScore64: return kernelFor<kBits, dev::kScore64>(threads);
-        case dev::kScoreTab:   // 8-bit words only
-            if constexpr (kBits == 8) return kernelFor<kBits, dev::kScoreTab>(threads);
+        case dev::kScore64: return kernelFor<kBits, dev::kScore64, kHbm>(threads);
+        case dev::kScoreTab:   // 8-bit words with LDS-resident state only
+            if constexpr (kBits == 8 && !kHbm) return kernelFor<kBits, dev::kScoreTab, kHbm>(threads);
             return nullptr;
         default: return nullptr;
     }
 }
 
-static const void* kernelFor(int bits, int mode, int threads) {
+static const void* kernelFor(int bits, int mode, int threads, bool hbm) {
     switch (bits) {
-        case 8:  return kernelForMode<8>(mode, threads);
-        case 16: return kernelForMode<16>(mode, threads);
-        case 32: return kernelForMode<32>(mode, threads);
+        case 8:  return hbm ? kernelForMode<8, true>(mode, threads) : kernelForMode<8, false>(mode, threads);
+        case 16: return hbm ? kernelForMode<16, true>(mode, threads) : kernelForMode<16, false>(mode, threads);
+        case 32: return hbm ? kernelForMode<32, true>(mode, threads) : kernelForMode<32, false>(mode, threads);
         default: return nullptr;
     }
 }
 
-hipError_t anneal(int bits, int mode, int threads, const ChainItem* items, int nItems, uint32_t* sched,
+hipError_t anneal(int bits, int mode, int threads, bool hbm, const ChainItem* items, int nItems, uint32_t* sched,
                   const LdsPlan& plan, int grid, uint64_t sliceSweeps, hipStream_t s) {
-    const void* kern = kernelFor(bits, mode, threads);
+    const void* kern = kernelFor(bits, mode, threads, hbm);
     if (!kern) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        static_cast<int>(plan.total));
@@ -1006,8 +1062,8 @@ hipError_t anneal(int bits, int mode, int threads, const ChainItem* items, int n
     return hipLaunchKernel(kern, dim3(grid), dim3(threads), args, plan.total, s);
 }
 
-int maxActiveBlocks(int bits, int mode, int threads, size_t lds) {
-    const void* kern = kernelFor(bits, mode, threads);
+int maxActiveBlocks(int bits, int mode, int threads, bool hbm, size_t lds) {
+    const void* kern = kernelFor(bits, mode, threads, hbm);
     if (!kern) return 0;
     if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) !=
         hipSuccess)

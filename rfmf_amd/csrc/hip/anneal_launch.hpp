@@ -66,8 +66,10 @@ struct InitItem {
 
 inline uint32_t alignUp(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
 
-inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads, int mode) {
+// hbm: the chain state stays in HBM, so the plan has no words or best bits.
+inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads, int mode, bool hbm = false) {
     LdsPlan p{};
+    if (hbm) maxN = 0;
     const int waves = threads / 64;
     const int laneP = maxP < kMaxLaneReduceP ? maxP : kMaxLaneReduceP;
     const int slots = (waves * laneP > maxP) ? waves * laneP : maxP;
@@ -96,12 +98,13 @@ namespace launch {
 // Persistent slice kernel over `nItems` chain items, each advanced by at most
 // `sliceSweeps` sweeps, `threads` threads per workgroup; `sched` is a 2-word
 // device counter pair, zero before the launch and left zero after it.
-hipError_t anneal(int bits, int mode, int threads, const dev::ChainItem* items, int nItems,
+// `hbm`: the chains' state stays in HBM (no LDS copy).
+hipError_t anneal(int bits, int mode, int threads, bool hbm, const dev::ChainItem* items, int nItems,
                   uint32_t* sched, const dev::LdsPlan& plan, int grid, uint64_t sliceSweeps,
                   hipStream_t s);
 
 // Resident workgroups per CU for a variant at a given dynamic LDS size.
-int maxActiveBlocks(int bits, int mode, int threads, size_t lds);
+int maxActiveBlocks(int bits, int mode, int threads, bool hbm, size_t lds);
 
 // Device-to-device copy of `bytes` (a multiple of 16) with a plain kernel.
 hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s);

@@ -353,7 +353,8 @@ rfmf_status rfmf_ctx_device_info(rfmf_ctx* ctx, int32_t* cu_count, int64_t* lds_
 }
 
 rfmf_status rfmf_ctx_last_launch(rfmf_ctx* ctx, int32_t* threads, int32_t* grid, int32_t* word_bits,
-                                 int32_t* occupancy, float* trial_ms_per_sweep, int32_t* score_mode) {
+                                 int32_t* occupancy, float* trial_ms_per_sweep, int32_t* score_mode,
+                                 int32_t* state_in_hbm) {
     return guarded([&] {
         if (!ctx) throw Error(Status::InvalidArg, "null context");
         std::lock_guard<std::mutex> lk(ctx->ctx->mutex());
@@ -363,6 +364,7 @@ rfmf_status rfmf_ctx_last_launch(rfmf_ctx* ctx, int32_t* threads, int32_t* grid,
         if (word_bits) *word_bits = r.bits;
         if (occupancy) *occupancy = r.occupancy;
         if (score_mode) *score_mode = r.mode;
+        if (state_in_hbm) *state_in_hbm = r.hbm ? 1 : 0;
         if (trial_ms_per_sweep)
             for (int i = 0; i < 4; ++i) trial_ms_per_sweep[i] = r.trialMsPerSweep[i];
     });
@@ -438,6 +440,9 @@ rfmf::LaunchKnobs knobsFrom(const rfmf_tuning* t) {
         if (t->score_table < -1 || t->score_table > 1)
             throw Error(Status::InvalidArg, "score_table must be -1, 0 or 1");
         k.scoreTable = t->score_table;
+        if (t->state_in_hbm < 0 || t->state_in_hbm > 1)
+            throw Error(Status::InvalidArg, "state_in_hbm must be 0 or 1");
+        k.stateInHbm = t->state_in_hbm;
     }
     return k;
 }

@@ -197,39 +197,45 @@ const DevGraph* Context::graph(int32_t id) const {
     return it->second.get();
 }
 
-int Context::occupancy(int bits, int mode, int threads, size_t lds) {
-    const uint64_t key = (static_cast<uint64_t>(bits) << 48) | (static_cast<uint64_t>(mode) << 44) |
-                         (static_cast<uint64_t>(threads) << 32) | lds;
+int Context::occupancy(int bits, int mode, int threads, size_t lds, bool hbm) {
+    const uint64_t key = (static_cast<uint64_t>(hbm) << 56) | (static_cast<uint64_t>(bits) << 48) |
+                         (static_cast<uint64_t>(mode) << 44) | (static_cast<uint64_t>(threads) << 32) | lds;
     const auto it = occCache_.find(key);
     if (it != occCache_.end()) return it->second;
-    const int occ = launch::maxActiveBlocks(bits, mode, threads, lds);
+    const int occ = launch::maxActiveBlocks(bits, mode, threads, hbm, lds);
     occCache_[key] = occ;
     return occ;
 }
 
-uint64_t Context::tuneKey(int bits, int mode, int64_t maxN, int64_t chains, int maxP) {
+uint64_t Context::tuneKey(int bits, int mode, int64_t maxN, int64_t chains, int maxP, bool hbm) {
     auto lg = [](int64_t x) { int l = 0; while ((int64_t{1} << (l + 1)) <= x) ++l; return static_cast<uint64_t>(l); };
-    return (static_cast<uint64_t>(bits) << 42) | (static_cast<uint64_t>(mode) << 40) |
+    return (static_cast<uint64_t>(hbm) << 48) | (static_cast<uint64_t>(bits) << 42) |
+           (static_cast<uint64_t>(mode) << 40) |
            (lg(std::max<int64_t>(maxN, 1)) << 32) | (lg(std::max<int64_t>(chains, 1)) << 24) |
            static_cast<uint64_t>(maxP);
 }
 
-bool Context::fitsLds(int64_t maxN, int bits, int mode, int maxP, int threads) const {
-    const dev::LdsPlan p = dev::makeLdsPlan(maxN, bits, maxP, threads, mode);
+bool Context::fitsLds(int64_t maxN, int bits, int mode, int maxP, int threads, bool hbm) const {
+    const dev::LdsPlan p = dev::makeLdsPlan(maxN, bits, maxP, threads, mode, hbm);
     return static_cast<size_t>(p.total) + dev::staticLds(mode) <= props_.sharedMemPerBlock;
 }
 
-int Context::chooseThreads(int bits, int mode, int64_t maxN, int64_t chains, int maxP) const {
-    const auto it = tuned_.find(tuneKey(bits, mode, maxN, chains, maxP));
+bool Context::stateInHbm(int64_t n, int bits, int mode, int maxP, const LaunchKnobs& knobs) const {
+    return knobs.stateInHbm > 0 || !fitsLds(n, bits, mode, maxP, 128);
+}
+
+int Context::chooseThreads(int bits, int mode, int64_t maxN, int64_t chains, int maxP, bool hbm) const {
+    const auto it = tuned_.find(tuneKey(bits, mode, maxN, chains, maxP, hbm));
     if (it != tuned_.end()) return it->second;
     // Heuristic: with several chains per CU the 4-wave tile keeps every SIMD
     // busy across chains; with about one chain per CU or fewer, 8 waves per
     // chain cut the per-sweep latency (16 waves add more per-wave work than
-    // they hide).
+    // they hide), and 16 when the state is in HBM, whose loads want more
+    // waves in flight.
     const double perCu = static_cast<double>(chains) / std::max(1, props_.multiProcessorCount);
-    int t = 512;
+    int t = hbm ? 1024 : 512;
     if (perCu >= 2.0) t = maxN <= 2048 ? 128 : 256;
-    while (t > 128 && !fitsLds(maxN, bits, mode, maxP, t)) t /= 2;
+    while (t > 128 && !fitsLds(maxN, bits, mode, maxP, t, hbm)) t /= 2;
     return t;
 }
 
@@ -322,12 +328,13 @@ HostJob Context::prepareJob(const JobSpec& spec, const HostGraph& g) const {
     else
         hj.mode = dev::kScore32;
     // The smallest tile (128 threads) has the smallest plan; launches pick a
-    // tile that fits.
-    const dev::LdsPlan plan = dev::makeLdsPlan(n, hj.bits, maxP, 128, hj.mode);
+    // tile that fits, and keep the chain state in HBM when none holds it.
+    // The group tables alone must fit.
+    const dev::LdsPlan plan = dev::makeLdsPlan(n, hj.bits, maxP, 128, hj.mode, true);
     if (static_cast<size_t>(plan.total) + dev::staticLds(hj.mode) > props_.sharedMemPerBlock)
-        throw Error(Status::Unsupported, "chain state of n=" + std::to_string(n) + " (" +
+        throw Error(Status::Unsupported, "group tables for P=" + std::to_string(maxP) + " (" +
                                              std::to_string(plan.total) +
-                                             " B) exceeds the LDS per workgroup");
+                                             " B) exceed the LDS per workgroup");
 
     for (int64_t i = 0; i < C * n; ++i)
         if (spec.initialSpins[i] != 1 && spec.initialSpins[i] != -1)
@@ -447,15 +454,18 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
                                                       const LaunchKnobs& knobs) {
     const int gridCap = knobs.gridCap;
     const int threads = knobs.threads;
-    // Group unfinished chains by kernel variant; items sorted by descending n
-    // so the persistent workgroups take the longest chains first.
-    constexpr int kGroups = 3 * dev::kScoreModes;
+    // Group unfinished chains by kernel variant (word bits, score mode, state
+    // in LDS or HBM); items sorted by descending n so the persistent
+    // workgroups take the longest chains first.
+    constexpr int kGroups = 2 * 3 * dev::kScoreModes;
     std::vector<LaunchGroup> groups(kGroups);
     std::vector<int64_t> maxN(kGroups, 0);
     std::vector<int> maxP(kGroups, 1);
-    auto gidx = [](int bits, int mode) { return (bits == 8 ? 0 : bits == 16 ? 1 : 2) * dev::kScoreModes + mode; };
+    auto gidx = [](bool hbm, int bits, int mode) {
+        return ((hbm ? 3 : 0) + (bits == 8 ? 0 : bits == 16 ? 1 : 2)) * dev::kScoreModes + mode;
+    };
     for (ResidentJob* j : jobs) {
-        const int gi = gidx(j->bits, j->mode);
+        const int gi = gidx(stateInHbm(j->graph->n, j->bits, j->mode, j->maxP, knobs), j->bits, j->mode);
         for (size_t c = 0; c < j->items.size(); ++c) {
             if (j->stepsDone[c] >= j->totalSteps) continue;
             groups[gi].items.push_back(j->items[c]);
@@ -469,18 +479,20 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
         if (g.items.empty()) continue;
         std::stable_sort(g.items.begin(), g.items.end(),
                          [](const dev::ChainItem& a, const dev::ChainItem& b) { return a.n > b.n; });
-        g.bits = (gi / dev::kScoreModes == 0) ? 8 : (gi / dev::kScoreModes == 1) ? 16 : 32;
+        const int wb = (gi / dev::kScoreModes) % 3;
+        g.hbm = gi / (3 * dev::kScoreModes) == 1;
+        g.bits = wb == 0 ? 8 : wb == 1 ? 16 : 32;
         g.mode = gi % dev::kScoreModes;
         g.threads = threads > 0 ? threads
                                 : chooseThreads(g.bits, g.mode, maxN[gi], static_cast<int64_t>(g.items.size()),
-                                                maxP[gi]);
+                                                maxP[gi], g.hbm);
         if (!dev::validThreads(g.threads))
             throw Error(Status::InvalidArg, "threads per chain must be 128, 256, 512 or 1024");
-        g.plan = dev::makeLdsPlan(maxN[gi], g.bits, maxP[gi], g.threads, g.mode);
+        g.plan = dev::makeLdsPlan(maxN[gi], g.bits, maxP[gi], g.threads, g.mode, g.hbm);
         if (static_cast<size_t>(g.plan.total) + dev::staticLds(g.mode) > props_.sharedMemPerBlock)
             throw Error(Status::Unsupported, "combined launch needs " + std::to_string(g.plan.total) +
                                                  " B of LDS per workgroup");
-        int occ = std::max(1, occupancy(g.bits, g.mode, g.threads, g.plan.total));
+        int occ = std::max(1, occupancy(g.bits, g.mode, g.threads, g.plan.total, g.hbm));
         const int64_t nItems = static_cast<int64_t>(g.items.size());
         auto residentOf = [&](int o) {
             int64_t r = static_cast<int64_t>(o) * props_.multiProcessorCount;
@@ -490,7 +502,7 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
         // 8-bit words with 24-bit scores: the table variant (one mad per
         // score, 8 KB more static LDS) replaces the per-sweep LUT when it
         // keeps every chain of the launch resident that the LUT variant would.
-        if (g.bits == 8 && g.mode == dev::kScore24 && knobs.scoreTable >= 0) {
+        if (g.bits == 8 && g.mode == dev::kScore24 && !g.hbm && knobs.scoreTable >= 0) {
             const dev::LdsPlan tp = dev::makeLdsPlan(maxN[gi], 8, maxP[gi], g.threads, dev::kScoreTab);
             if (static_cast<size_t>(tp.total) + dev::staticLds(dev::kScoreTab) <= props_.sharedMemPerBlock) {
                 const int occTab = occupancy(8, dev::kScoreTab, g.threads, tp.total);
@@ -507,6 +519,7 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
         last_.bits = g.bits;
         last_.occupancy = occ;
         last_.mode = g.mode;
+        last_.hbm = g.hbm;
         out.push_back(std::move(g));
     }
     return out;
@@ -542,7 +555,7 @@ Context::SliceHandle Context::launchSlice(const std::vector<ResidentJob*>& jobs,
         at = 0;
         int ctr = 0;
         for (LaunchGroup& g : groups) {
-            hipCheck(launch::anneal(g.bits, g.mode, g.threads, slot.dev.as<dev::ChainItem>() + at,
+            hipCheck(launch::anneal(g.bits, g.mode, g.threads, g.hbm, slot.dev.as<dev::ChainItem>() + at,
                                     static_cast<int>(g.items.size()), sched_.as<uint32_t>() + 2 * ctr, g.plan,
                                     g.grid, slice, stream_), "annealSlice launch");
             at += g.items.size();
@@ -575,14 +588,15 @@ std::unique_ptr<Context::GraphedSlices> Context::captureSlices(const std::vector
         g->itemBufs.emplace_back(lg.items.size() * sizeof(dev::ChainItem));
         hipCheck(hipMemcpy(g->itemBufs.back().get(), lg.items.data(), g->itemBufs.back().bytes(),
                            hipMemcpyHostToDevice), "upload items");
-        (void)launch::maxActiveBlocks(lg.bits, lg.mode, lg.threads, lg.plan.total);   // LDS attribute, outside the capture
+        (void)launch::maxActiveBlocks(lg.bits, lg.mode, lg.threads, lg.hbm, lg.plan.total);   // LDS attribute, outside the capture
     }
     hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     int slot = 0;
     hipError_t err = hipSuccess;
     for (size_t i = 0; i < groups.size() && err == hipSuccess; ++i, ++slot)
-        err = launch::anneal(groups[i].bits, groups[i].mode, groups[i].threads, g->itemBufs[i].as<dev::ChainItem>(),
+        err = launch::anneal(groups[i].bits, groups[i].mode, groups[i].threads, groups[i].hbm,
+                             g->itemBufs[i].as<dev::ChainItem>(),
                              static_cast<int>(groups[i].items.size()), sched_.as<uint32_t>() + 2 * slot,
                              groups[i].plan, groups[i].grid, sliceSweeps, stream_);
     const hipError_t endErr = hipStreamEndCapture(stream_, &g->graph);
@@ -626,8 +640,11 @@ float Context::waitSlice(SliceHandle& h) {
 int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* spentMs, int keyP,
                       uint64_t budget) {
     // Shape of the launch: widest word and score, largest n and P, unfinished chains.
+    // A candidate tile must fit the LDS-resident chains' plan and, if any
+    // chain keeps its state in HBM, the HBM plan.
     int bits = 8, mode = 0, maxP = 1;
-    int64_t maxN = 0, chains = 0;
+    int64_t maxN = 0, maxNLds = -1, chains = 0;
+    bool hbm = false;
     uint64_t minLeft = ~uint64_t{0};
     for (const ResidentJob* j : jobs)
         for (size_t c = 0; c < j->items.size(); ++c) {
@@ -636,11 +653,15 @@ int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& 
             mode = std::max(mode, j->mode);
             maxN = std::max<int64_t>(maxN, j->graph->n);
             maxP = std::max(maxP, j->maxP);
+            if (stateInHbm(j->graph->n, j->bits, j->mode, j->maxP, knobs))
+                hbm = true;
+            else
+                maxNLds = std::max<int64_t>(maxNLds, j->graph->n);
             ++chains;
             minLeft = std::min(minLeft, j->totalSteps - j->stepsDone[c]);
         }
     if (chains == 0) return 0;
-    const uint64_t key = tuneKey(bits, mode, maxN, chains, keyP > 0 ? keyP : maxP);
+    const uint64_t key = tuneKey(bits, mode, maxN, chains, keyP > 0 ? keyP : maxP, hbm);
     const auto hit = tuned_.find(key);
     last_.tuned = true;
     if (hit != tuned_.end()) return hit->second;
@@ -656,7 +677,8 @@ int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& 
     float bestPerSweep = 0.0f;
     for (int i = 0; i < 4; ++i) {
         const int t = dev::kThreadChoices[i];
-        if (!fitsLds(maxN, bits, mode, maxP, t)) continue;
+        if (maxNLds >= 0 && !fitsLds(maxNLds, bits, mode, maxP, t)) continue;
+        if (hbm && !fitsLds(0, bits, mode, maxP, t, true)) continue;
         LaunchKnobs k = knobs;
         k.threads = t;
         k.sliceSweeps = static_cast<int64_t>(trial);

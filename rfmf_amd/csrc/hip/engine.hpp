@@ -161,6 +161,7 @@ struct LaunchKnobs {
     int threads = 0;           // threads per chain (128/256/512/1024); 0: tuned or heuristic
     bool autotune = false;     // time the candidates on the run's first slices
     int scoreTable = 0;        // 8-bit words: 0 table when it costs no resident chains, 1 always, -1 never
+    int stateInHbm = 0;        // 0: only chains whose state exceeds LDS keep it in HBM; 1: every chain
 };
 
 // What the last launch of a context used (for reports).
@@ -170,6 +171,7 @@ struct LaunchReport {
     int bits = 0;
     int occupancy = 0;
     int mode = 0;   // dev::ScoreMode of the last group
+    bool hbm = false;   // the last group's state stayed in HBM
     bool tuned = false;
     float trialMsPerSweep[4] = {0, 0, 0, 0};   // by kThreadChoices, 0 if not tried
 };
@@ -256,10 +258,13 @@ public:
     void collect(const ResidentJob& job, const JobSpec& spec, hipStream_t stream);
 
     // Resident workgroups per CU of a kernel variant (cached).
-    int occupancy(int bits, int mode, int threads, size_t lds);
+    int occupancy(int bits, int mode, int threads, size_t lds, bool hbm = false);
     // Threads per chain for a launch shape: the autotuner's cached choice,
     // else a heuristic from chains per CU and n.
-    int chooseThreads(int bits, int mode, int64_t maxN, int64_t chains, int maxP) const;
+    int chooseThreads(int bits, int mode, int64_t maxN, int64_t chains, int maxP, bool hbm = false) const;
+    // Whether a chain of n vertices keeps its state in HBM: when its LDS plan
+    // does not fit even the smallest tile, or when the knobs ask for it.
+    bool stateInHbm(int64_t n, int bits, int mode, int maxP, const LaunchKnobs& knobs) const;
     const LaunchReport& lastLaunch() const { return last_; }
 
     std::mutex& mutex() { return mu_; }
@@ -272,10 +277,11 @@ private:
         int threads = dev::kThreads;
         dev::LdsPlan plan{};
         int grid = 1;
+        bool hbm = false;
     };
     std::vector<LaunchGroup> planGroups(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs);
-    static uint64_t tuneKey(int bits, int mode, int64_t maxN, int64_t chains, int maxP);
-    bool fitsLds(int64_t maxN, int bits, int mode, int maxP, int threads) const;
+    static uint64_t tuneKey(int bits, int mode, int64_t maxN, int64_t chains, int maxP, bool hbm);
+    bool fitsLds(int64_t maxN, int bits, int mode, int maxP, int threads, bool hbm = false) const;
     std::unordered_map<uint64_t, int> tuned_;
     LaunchReport last_;
 

@@ -120,6 +120,21 @@ def test_score_table_choice_invariance(graphs, score_table, mode):
     assert np.array_equal(torus.best_vals, bv)
 
 
+@pytest.mark.parametrize("slice_sweeps,use_graph", [(0, False), (37, True)])
+def test_state_in_hbm_invariance(graphs, slice_sweeps, use_graph):
+    # Every chain working on its state in place in HBM (the path of graphs too
+    # large for LDS) gives the same chains as the LDS-resident launches.
+    print(f"[batch] state_in_hbm slice={slice_sweeps} hipGraph={use_graph}", flush=True)
+    jobs = _mixed_jobs(graphs)
+    with R.Context(0) as ctx:
+        ids = {k: ctx.upload_graph(g) for k, g in graphs.items()}
+        ctx.batch_run([(ids[k], j) for k, j in jobs], slice_sweeps=slice_sweeps, use_hip_graph=use_graph,
+                      state_in_hbm=True)
+        assert ctx.last_launch()["state_in_hbm"]
+    for k, j in jobs:
+        same(j, one_shot(j), f"state_in_hbm {k}")
+
+
 def test_chain_isolation_beyond_resident_capacity(gset):
     # 3000 chains of an 800-vertex graph exceed the resident workgroups of
     # 256 CUs, so the persistent workgroups take several chains each.

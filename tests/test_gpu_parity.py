@@ -183,3 +183,61 @@ def test_int64_scores_equal_int32_scores_when_forced():
     wide = _case("forced-int64", csr, [1, 2, 3, 4], beta, phases, iters, B=2**29)
     assert np.array_equal(narrow.end_spins, wide.end_spins)
     assert np.array_equal(narrow.best_spins, wide.best_spins)
+
+
+# ---------------------------------------------------------- state beyond LDS
+# A chain whose state words do not fit one workgroup's LDS keeps them in HBM
+# and updates them in place (the reference keeps every chain's spins and
+# fields in global memory, src/anneal_kernels.cu:221-261); the results must
+# not change.  n = 180000 at 8-bit words, 90000 at 16-bit, 45000 at 32-bit.
+
+def _cycle_with_chords(n, wmax, seed):
+    rng = np.random.default_rng(seed)
+    u = np.arange(n)
+    v = (u + 1) % n
+    cu = rng.integers(0, n, n // 4)
+    cv = (cu + rng.integers(2, n - 1, cu.size)) % n
+    u, v = np.concatenate([u, cu]), np.concatenate([v, cv])
+    w = rng.integers(1, wmax + 1, u.size) * np.where(rng.random(u.size) < 0.5, -1, 1)
+    return R.graph_from_edges(u, v, w)
+
+
+def _hbm_case(name, csr, seeds, beta, phases, iters, bits):
+    print(f"[gpu-parity] {name}: n={csr.n} m2={csr.m2} chains={len(seeds)} phases={list(phases)} "
+          f"iters={list(iters)}", flush=True)
+    init = np.stack([R.generate_random_spins(s, csr.n) for s in seeds])
+    solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+    job = R.Job(csr, init, solver, beta, phases, iters)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ctx.batch_run([(gid, job)])
+        geo = ctx.last_launch()
+    assert geo["state_in_hbm"] and geo["word_bits"] == bits, geo
+    end, best, cv, bv = run_oracle_chains(csr, init, solver, beta, phases, iters)
+    assert np.array_equal(job.end_spins, end), f"{name}: final spins differ"
+    assert np.array_equal(job.best_spins, best), f"{name}: best spins differ"
+    assert np.array_equal(job.final_vals, cv) and np.array_equal(job.best_vals, bv), f"{name}: cuts differ"
+    for c in range(len(seeds)):
+        assert R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c]
+
+
+def test_state_beyond_lds_u8_words():
+    csr = W.torus(400, 450, seed=9)
+    phases, iters = [128, 8, 1], [30, 30, 20]
+    _hbm_case("torus-180k", csr, [1, 2], R.build_linear_beta_table(0.5, 3.0, sum(iters), 2), phases, iters, 8)
+
+
+def test_state_beyond_lds_u16_words():
+    csr = _cycle_with_chords(90000, 40, seed=13)
+    assert 63 < R.max_weighted_degree(csr) <= 16383
+    phases, iters = [64, 3, 1], [30, 30, 20]
+    _hbm_case("cycle-90k-u16", csr, [3, 4], R.build_linear_beta_table(0.0, 0.2, sum(iters), 2), phases, iters,
+              16)
+
+
+def test_state_beyond_lds_u32_words():
+    csr = _cycle_with_chords(45000, 9000, seed=14)
+    assert R.max_weighted_degree(csr) > 16383
+    phases, iters = [32, 1], [40, 40]
+    _hbm_case("cycle-45k-u32", csr, [5, 6], R.build_linear_beta_table(0.0, 0.001, sum(iters), 2), phases, iters,
+              32)

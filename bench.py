@@ -74,7 +74,8 @@ def main() -> int:
     ap.add_argument("--cpu-sweeps", type=int, default=64000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (counter calibration) and exit")
-    ap.add_argument("--config", default="torus", choices=["torus", "er800", "sparse", "batch", "serve", "gset"],
+    ap.add_argument("--config", default="torus",
+                    choices=["torus", "er800", "sparse", "batch", "serve", "gset", "large"],
                     help="torus: the headline plain run (default); the others are the other workloads")
     ap.add_argument("--threads", type=int, default=0, help="threads per chain (0: autotune)")
     ap.add_argument("--qps", type=float, default=20.0, help="serve: Poisson arrival rate")
@@ -84,7 +85,7 @@ def main() -> int:
 
     if args.config != "torus" and not args.copy_probe:
         return {"er800": bench_er800, "sparse": bench_sparse, "batch": bench_batch,
-                "serve": bench_serve, "gset": bench_gset}[args.config](args)
+                "serve": bench_serve, "gset": bench_gset, "large": bench_large}[args.config](args)
 
     if args.copy_probe:
         import rfmf_amd as R
@@ -425,6 +426,37 @@ def bench_gset(args) -> int:
                "value": rate, "us_per_sweep": ms * 1e3 / sum(iters), "reference_v100_sxm2": V100_SXM2[inst],
                "ratio_vs_reference": rate / V100_SXM2[inst], "best_cut_max": int(job.best_vals.max()),
                "cuts_recomputed": bool(ok)})
+    return rc
+
+
+def bench_large(args) -> int:
+    """A graph whose chain state does not fit LDS: torus 1000x1000 +/-1
+    (n = 10^6, 1 MB of 8-bit words per chain), 256 chains x 800 sweeps,
+    ladder 128..1, linear beta: the chains work on their state in place in
+    HBM.  Beside it, the headline torus 100x200 (1024 chains x 8000 sweeps)
+    with every chain forced to HBM-resident state, against its LDS-resident
+    rate.  Geometry from the built-in heuristic (runs too short for the
+    autotuner's trial slices)."""
+    import rfmf_amd as R
+    from rfmf_amd import workloads as W
+    rc = 0
+    for name, csr, C, sweeps, kw in (
+            ("torus1000x1000_256chains", W.torus(1000, 1000, seed=1), 256, 800, {}),
+            ("torus100x200_1024chains_lds", W.torus(100, 200, seed=0), 1024, 8000, {}),
+            ("torus100x200_1024chains_hbm", W.torus(100, 200, seed=0), 1024, 8000, {"state_in_hbm": True})):
+        iters = [sweeps // 8] * 8
+        job = _job(R, csr, range(1, C + 1), iters)
+        with R.Context(0) as ctx:
+            gid = ctx.upload_graph(csr)
+            ms = ctx.batch_run([(gid, job)], threads=args.threads, **kw)
+            geo = ctx.last_launch()
+        ok = all(R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c] for c in range(0, C, 64))
+        rc |= 0 if ok else 1
+        rate = C * sum(iters) / (ms / 1e3)
+        log(f"[large] {name}: n={csr.n} {ms:.1f} ms {rate:.3e} chain-steps/s {rate * csr.n:.3e} vertex-steps/s "
+            f"({geo})")
+        _emit({"metric": "chain-steps/sec", "config": name, "n": csr.n, "chains": C, "sweeps": sum(iters),
+               "value": rate, "vertex_steps_per_s": rate * csr.n, "geometry": geo, "cuts_recomputed": bool(ok)})
     return rc
 
 

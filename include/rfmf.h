@@ -21,7 +21,7 @@ extern "C" {
 typedef enum rfmf_status {
     RFMF_OK = 0,
     RFMF_ERR_INVALID_ARG = 1,   /* sizes, null pointers, t0 <= 0, P out of range */
-    RFMF_ERR_UNSUPPORTED = 2,   /* max weighted degree >= 2^30, chain state beyond LDS, P > 4096 */
+    RFMF_ERR_UNSUPPORTED = 2,   /* max weighted degree >= 2^30, P > 4096 (group tables beyond LDS) */
     RFMF_ERR_HIP = 3,           /* HIP runtime error (message has the detail) */
     RFMF_ERR_NO_DEVICE = 4,
     RFMF_ERR_OUT_OF_MEMORY = 5,

@@ -228,13 +228,14 @@ int Context::chooseThreads(int bits, int mode, int64_t maxN, int64_t chains, int
     const auto it = tuned_.find(tuneKey(bits, mode, maxN, chains, maxP, hbm));
     if (it != tuned_.end()) return it->second;
     // Heuristic: with several chains per CU the 4-wave tile keeps every SIMD
-    // busy across chains; with about one chain per CU or fewer, 8 waves per
-    // chain cut the per-sweep latency (16 waves add more per-wave work than
-    // they hide), and 16 when the state is in HBM, whose loads want more
-    // waves in flight.
+    // busy across chains (the 2-wave tile for small graphs, and for state in
+    // HBM, where LDS no longer limits the chains per CU); with about one chain
+    // per CU or fewer, 8 waves per chain cut the per-sweep latency (16 waves
+    // add more per-wave work than they hide), and 16 when the state is in
+    // HBM, whose loads want more waves in flight.
     const double perCu = static_cast<double>(chains) / std::max(1, props_.multiProcessorCount);
     int t = hbm ? 1024 : 512;
-    if (perCu >= 2.0) t = maxN <= 2048 ? 128 : 256;
+    if (perCu >= 2.0) t = (hbm || maxN <= 2048) ? 128 : 256;
     while (t > 128 && !fitsLds(maxN, bits, mode, maxP, t, hbm)) t /= 2;
     return t;
 }

@@ -1004,7 +1004,6 @@ namespace launch {
 using dev::ChainItem;
 using dev::InitItem;
 using dev::LdsPlan;
-using dev::kThreads;
 
 hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s) {
     const size_t n16 = bytes / 16;

@@ -45,7 +45,6 @@ struct LdsPlan {
     uint32_t pickOff;     // int32 winner id per group
     uint32_t spinOff;     // int32 winner pre-flip spin per group
     uint32_t deltaOff;    // int32 winner pre-flip delta per group
-    uint32_t lutOff;      // unused (the LUT is static LDS)
     uint32_t dpartOff;    // int64 per-wave cut-delta partials
     uint32_t miscOff;     // scheduler broadcast
     uint32_t total;       // dynamic LDS bytes
@@ -74,7 +73,6 @@ inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads, int mo
     const int laneP = maxP < kMaxLaneReduceP ? maxP : kMaxLaneReduceP;
     const int slots = (waves * laneP > maxP) ? waves * laneP : maxP;
     uint32_t off = 0;
-    p.lutOff = 0;     // the Gumbel LUT is a static 64 B LDS array of the kernel
     p.wordsOff = off; off += alignUp(static_cast<uint32_t>(wordBytes(maxN, bits)), 16);
     p.bestOff = off;  off += alignUp(static_cast<uint32_t>(bestBytes(maxN)), 16);
     p.partOff = off;  off += alignUp(static_cast<uint32_t>(slots) * 8u, 16);

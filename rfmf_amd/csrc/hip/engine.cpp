@@ -78,7 +78,10 @@ Context::Context(int device) : device_(device) {
     hipCheck(hipGetDeviceProperties(&props_, device_), "hipGetDeviceProperties");
     hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hipCheck(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreate");
-    sched_.alloc(64 * sizeof(uint32_t));
+    for (hipStream_t& l : lanes_) hipCheck(hipStreamCreateWithFlags(&l, hipStreamNonBlocking), "hipStreamCreate");
+    hipCheck(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreate");
+    for (hipEvent_t& e : join_) hipCheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    sched_.alloc(2 * kMaxGroups * sizeof(uint32_t));
     hipCheck(hipMemset(sched_.get(), 0, sched_.bytes()), "hipMemset");
 }
 
@@ -86,6 +89,8 @@ Context::~Context() {
     (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (side_) (void)hipStreamSynchronize(side_);
+    for (hipStream_t l : lanes_)
+        if (l) (void)hipStreamSynchronize(l);
     graphs_.clear();
     sched_.reset();
     for (ItemSlot& s : slots_) {
@@ -95,6 +100,11 @@ Context::~Context() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (side_) (void)hipStreamSynchronize(side_);
     if (pool_) (void)hipMemPoolDestroy(pool_);
+    for (hipStream_t l : lanes_)
+        if (l) (void)hipStreamDestroy(l);
+    if (fork_) (void)hipEventDestroy(fork_);
+    for (hipEvent_t e : join_)
+        if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
     if (side_) (void)hipStreamDestroy(side_);
 }
@@ -451,86 +461,167 @@ float Context::runSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
     return waitSlice(h);
 }
 
+void Context::finishGroup(LaunchGroup& g, int64_t maxN, int maxP, const LaunchKnobs& knobs) {
+    // Tile, LDS plan, score arithmetic and grid for one launch group.
+    g.threads = knobs.threads > 0
+                    ? knobs.threads
+                    : chooseThreads(g.bits, g.mode, maxN, static_cast<int64_t>(g.items.size()), maxP, g.hbm);
+    if (!dev::validThreads(g.threads))
+        throw Error(Status::InvalidArg, "threads per chain must be 128, 256, 512 or 1024");
+    g.plan = dev::makeLdsPlan(maxN, g.bits, maxP, g.threads, g.mode, g.hbm);
+    if (static_cast<size_t>(g.plan.total) + dev::staticLds(g.mode) > props_.sharedMemPerBlock)
+        throw Error(Status::Unsupported, "combined launch needs " + std::to_string(g.plan.total) +
+                                             " B of LDS per workgroup");
+    int occ = std::max(1, occupancy(g.bits, g.mode, g.threads, g.plan.total, g.hbm));
+    const int64_t nItems = static_cast<int64_t>(g.items.size());
+    auto residentOf = [&](int o) {
+        int64_t r = static_cast<int64_t>(o) * props_.multiProcessorCount;
+        if (knobs.gridCap > 0) r = std::min<int64_t>(r, knobs.gridCap);
+        return std::min(r, nItems);
+    };
+    // 8-bit words with 24-bit scores: the table variant (one mad per score,
+    // 8 KB more static LDS) replaces the per-sweep LUT when it keeps every
+    // chain of the group resident that the LUT variant would.
+    if (g.bits == 8 && g.mode == dev::kScore24 && !g.hbm && knobs.scoreTable >= 0) {
+        const dev::LdsPlan tp = dev::makeLdsPlan(maxN, 8, maxP, g.threads, dev::kScoreTab);
+        if (static_cast<size_t>(tp.total) + dev::staticLds(dev::kScoreTab) <= props_.sharedMemPerBlock) {
+            const int occTab = occupancy(8, dev::kScoreTab, g.threads, tp.total);
+            if (occTab > 0 && (knobs.scoreTable > 0 || residentOf(occTab) >= residentOf(occ))) {
+                g.mode = dev::kScoreTab;
+                g.plan = tp;
+                occ = occTab;
+            }
+        }
+    }
+    g.occupancy = occ;
+    g.grid = static_cast<int>(residentOf(occ));
+}
+
 std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<ResidentJob*>& jobs,
-                                                      const LaunchKnobs& knobs) {
-    const int gridCap = knobs.gridCap;
-    const int threads = knobs.threads;
+                                                      const LaunchKnobs& knobs, uint64_t slice) {
     // Group unfinished chains by kernel variant (word bits, score mode, state
     // in LDS or HBM); items sorted by descending n so the persistent
     // workgroups take the longest chains first.
-    constexpr int kGroups = 2 * 3 * dev::kScoreModes;
-    std::vector<LaunchGroup> groups(kGroups);
-    std::vector<int64_t> maxN(kGroups, 0);
-    std::vector<int> maxP(kGroups, 1);
+    constexpr int kVariants = 2 * 3 * dev::kScoreModes;
+    std::vector<LaunchGroup> base(kVariants);
+    std::vector<int> maxP(kVariants, 1);
     auto gidx = [](bool hbm, int bits, int mode) {
         return ((hbm ? 3 : 0) + (bits == 8 ? 0 : bits == 16 ? 1 : 2)) * dev::kScoreModes + mode;
     };
     for (ResidentJob* j : jobs) {
-        const int gi = gidx(stateInHbm(j->graph->n, j->bits, j->mode, j->maxP, knobs), j->bits, j->mode);
+        const bool hbm = stateInHbm(j->graph->n, j->bits, j->mode, j->maxP, knobs);
+        const int gi = gidx(hbm, j->bits, j->mode);
+        base[gi].hbm = hbm;
+        base[gi].bits = j->bits;
+        base[gi].mode = j->mode;
         for (size_t c = 0; c < j->items.size(); ++c) {
             if (j->stepsDone[c] >= j->totalSteps) continue;
-            groups[gi].items.push_back(j->items[c]);
-            maxN[gi] = std::max<int64_t>(maxN[gi], j->graph->n);
+            base[gi].items.push_back(j->items[c]);
             maxP[gi] = std::max(maxP[gi], j->maxP);
         }
     }
+    // Size classes: a variant's chains split at n = 2^k (k >= 11) wherever
+    // the smaller chains get a different tile, occupancy or score variant
+    // from a plan sized for themselves (adjacent classes with the same
+    // launch shape merge).  Small graphs in a mixed batch then run at the
+    // occupancy their own LDS plan allows instead of the largest graph's.
+    // Only for long launches: each class is a launch of its own, whose start
+    // and end cost more than the occupancy gains on short slices (the
+    // serving loop's); and a grid cap keeps one group per variant.
+    const bool split = knobs.gridCap <= 0 && slice >= kClassMinSweeps;
     std::vector<LaunchGroup> out;
-    for (int gi = 0; gi < kGroups; ++gi) {
-        LaunchGroup& g = groups[gi];
-        if (g.items.empty()) continue;
-        std::stable_sort(g.items.begin(), g.items.end(),
-                         [](const dev::ChainItem& a, const dev::ChainItem& b) { return a.n > b.n; });
-        const int wb = (gi / dev::kScoreModes) % 3;
-        g.hbm = gi / (3 * dev::kScoreModes) == 1;
-        g.bits = wb == 0 ? 8 : wb == 1 ? 16 : 32;
-        g.mode = gi % dev::kScoreModes;
-        g.threads = threads > 0 ? threads
-                                : chooseThreads(g.bits, g.mode, maxN[gi], static_cast<int64_t>(g.items.size()),
-                                                maxP[gi], g.hbm);
-        if (!dev::validThreads(g.threads))
-            throw Error(Status::InvalidArg, "threads per chain must be 128, 256, 512 or 1024");
-        g.plan = dev::makeLdsPlan(maxN[gi], g.bits, maxP[gi], g.threads, g.mode, g.hbm);
-        if (static_cast<size_t>(g.plan.total) + dev::staticLds(g.mode) > props_.sharedMemPerBlock)
-            throw Error(Status::Unsupported, "combined launch needs " + std::to_string(g.plan.total) +
-                                                 " B of LDS per workgroup");
-        int occ = std::max(1, occupancy(g.bits, g.mode, g.threads, g.plan.total, g.hbm));
-        const int64_t nItems = static_cast<int64_t>(g.items.size());
-        auto residentOf = [&](int o) {
-            int64_t r = static_cast<int64_t>(o) * props_.multiProcessorCount;
-            if (gridCap > 0) r = std::min<int64_t>(r, gridCap);
-            return std::min(r, nItems);
-        };
-        // 8-bit words with 24-bit scores: the table variant (one mad per
-        // score, 8 KB more static LDS) replaces the per-sweep LUT when it
-        // keeps every chain of the launch resident that the LUT variant would.
-        if (g.bits == 8 && g.mode == dev::kScore24 && !g.hbm && knobs.scoreTable >= 0) {
-            const dev::LdsPlan tp = dev::makeLdsPlan(maxN[gi], 8, maxP[gi], g.threads, dev::kScoreTab);
-            if (static_cast<size_t>(tp.total) + dev::staticLds(dev::kScoreTab) <= props_.sharedMemPerBlock) {
-                const int occTab = occupancy(8, dev::kScoreTab, g.threads, tp.total);
-                if (occTab > 0 && (knobs.scoreTable > 0 || residentOf(occTab) >= residentOf(occ))) {
-                    g.mode = dev::kScoreTab;
-                    g.plan = tp;
-                    occ = occTab;
+    for (int gi = 0; gi < kVariants; ++gi) {
+        LaunchGroup& b = base[gi];
+        if (b.items.empty()) continue;
+        std::stable_sort(b.items.begin(), b.items.end(),
+                         [](const dev::ChainItem& x, const dev::ChainItem& y) { return x.n > y.n; });
+        std::vector<LaunchGroup> classes;
+        size_t at = 0;
+        while (at < b.items.size()) {
+            const int64_t top = b.items[at].n;
+            int64_t floorN = 0;   // the class holds n in (floorN, top]
+            if (split)
+                for (int64_t edge = int64_t{1} << 11; edge < top; edge <<= 1) floorN = edge;
+            size_t stop = at;
+            while (stop < b.items.size() && b.items[stop].n > floorN) ++stop;
+            LaunchGroup c;
+            c.hbm = b.hbm;
+            c.bits = b.bits;
+            c.mode = b.mode;
+            c.items.assign(b.items.begin() + static_cast<std::ptrdiff_t>(at),
+                           b.items.begin() + static_cast<std::ptrdiff_t>(stop));
+            finishGroup(c, top, maxP[gi], knobs);
+            if (!classes.empty()) {
+                LaunchGroup& prev = classes.back();
+                if (prev.threads == c.threads && prev.mode == c.mode && prev.occupancy == c.occupancy) {
+                    // Re-plan the merged class for its item count (the grid
+                    // grows with it).
+                    prev.items.insert(prev.items.end(), c.items.begin(), c.items.end());
+                    prev.mode = b.mode;
+                    finishGroup(prev, prev.items.front().n, maxP[gi], knobs);
+                    at = stop;
+                    continue;
                 }
             }
+            classes.push_back(std::move(c));
+            at = stop;
         }
-        g.grid = static_cast<int>(residentOf(occ));
+        for (LaunchGroup& c : classes) out.push_back(std::move(c));
+    }
+    // Largest work first: its launch starts first, and the others, on
+    // concurrent lanes, fill its tail.
+    auto work = [](const LaunchGroup& g) {
+        int64_t w = 0;
+        for (const dev::ChainItem& it : g.items) w += it.n;
+        return w;
+    };
+    std::stable_sort(out.begin(), out.end(),
+                     [&](const LaunchGroup& x, const LaunchGroup& y) { return work(x) > work(y); });
+    if (out.size() > kMaxGroups)
+        throw Error(Status::Unsupported, "a launch needs more than " + std::to_string(kMaxGroups) + " groups");
+    if (!out.empty()) {
+        const LaunchGroup& g = out.front();
         last_.threads = g.threads;
         last_.grid = g.grid;
         last_.bits = g.bits;
-        last_.occupancy = occ;
+        last_.occupancy = g.occupancy;
         last_.mode = g.mode;
         last_.hbm = g.hbm;
-        out.push_back(std::move(g));
     }
     return out;
+}
+
+hipError_t Context::enqueueGroups(const std::vector<LaunchGroup>& groups,
+                                  const std::vector<dev::ChainItem*>& items, uint64_t slice) {
+    // Group i runs on lane i mod kLanes (lane 0 is stream_; the others fork
+    // from it and join back), with its own work-counter pair.  Every lane is
+    // joined even after an error, so a capture can always end.
+    const int lanes = static_cast<int>(std::min<size_t>(kLanes, groups.size()));
+    hipError_t err = hipSuccess;
+    auto keep = [&](hipError_t e) { if (err == hipSuccess) err = e; };
+    if (lanes > 1) {
+        keep(hipEventRecord(fork_, stream_));
+        for (int l = 1; l < lanes; ++l) keep(hipStreamWaitEvent(lanes_[l - 1], fork_, 0));
+    }
+    for (size_t i = 0; i < groups.size() && err == hipSuccess; ++i) {
+        const LaunchGroup& g = groups[i];
+        const size_t l = i % kLanes;
+        keep(launch::anneal(g.bits, g.mode, g.threads, g.hbm, items[i], static_cast<int>(g.items.size()),
+                            sched_.as<uint32_t>() + 2 * i, g.plan, g.grid, slice,
+                            l == 0 ? stream_ : lanes_[l - 1]));
+    }
+    for (int l = 1; l < lanes; ++l) {
+        keep(hipEventRecord(join_[l - 1], lanes_[l - 1]));
+        keep(hipStreamWaitEvent(stream_, join_[l - 1], 0));
+    }
+    return err;
 }
 
 Context::SliceHandle Context::launchSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs) {
     hipCheck(hipSetDevice(device_), "hipSetDevice");
     SliceHandle h;
-    std::vector<LaunchGroup> groups = planGroups(jobs, knobs);
     const uint64_t slice = knobs.sliceSweeps > 0 ? static_cast<uint64_t>(knobs.sliceSweeps) : ~uint64_t{0};
+    std::vector<LaunchGroup> groups = planGroups(jobs, knobs, slice);
     size_t nItems = 0;
     for (const LaunchGroup& g : groups) nItems += g.items.size();
     if (nItems > 0) {
@@ -553,15 +644,13 @@ Context::SliceHandle Context::launchSlice(const std::vector<ResidentJob*>& jobs,
         hipCheck(hipEventCreate(&h.ev1), "hipEventCreate");
         hipCheck(hipEventRecord(h.ev0, stream_), "hipEventRecord");
         hipCheck(hipMemcpyAsync(slot.dev.get(), staged, bytes, hipMemcpyHostToDevice, stream_), "upload items");
+        std::vector<dev::ChainItem*> ptrs;
         at = 0;
-        int ctr = 0;
-        for (LaunchGroup& g : groups) {
-            hipCheck(launch::anneal(g.bits, g.mode, g.threads, g.hbm, slot.dev.as<dev::ChainItem>() + at,
-                                    static_cast<int>(g.items.size()), sched_.as<uint32_t>() + 2 * ctr, g.plan,
-                                    g.grid, slice, stream_), "annealSlice launch");
+        for (const LaunchGroup& g : groups) {
+            ptrs.push_back(slot.dev.as<dev::ChainItem>() + at);
             at += g.items.size();
-            ++ctr;
         }
+        hipCheck(enqueueGroups(groups, ptrs, slice), "annealSlice launch");
         hipCheck(hipEventRecord(slot.done, stream_), "hipEventRecord");
         hipCheck(hipEventRecord(h.ev1, stream_), "hipEventRecord");
         h.empty = false;
@@ -583,7 +672,7 @@ std::unique_ptr<Context::GraphedSlices> Context::captureSlices(const std::vector
     auto g = std::make_unique<GraphedSlices>();
     g->jobs = jobs;
     g->slice = sliceSweeps;
-    std::vector<LaunchGroup> groups = planGroups(jobs, knobs);
+    std::vector<LaunchGroup> groups = planGroups(jobs, knobs, sliceSweeps);
     // Item arrays go up before the capture; the graph holds kernel nodes only.
     for (LaunchGroup& lg : groups) {
         g->itemBufs.emplace_back(lg.items.size() * sizeof(dev::ChainItem));
@@ -593,13 +682,9 @@ std::unique_ptr<Context::GraphedSlices> Context::captureSlices(const std::vector
     }
     hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    int slot = 0;
-    hipError_t err = hipSuccess;
-    for (size_t i = 0; i < groups.size() && err == hipSuccess; ++i, ++slot)
-        err = launch::anneal(groups[i].bits, groups[i].mode, groups[i].threads, groups[i].hbm,
-                             g->itemBufs[i].as<dev::ChainItem>(),
-                             static_cast<int>(groups[i].items.size()), sched_.as<uint32_t>() + 2 * slot,
-                             groups[i].plan, groups[i].grid, sliceSweeps, stream_);
+    std::vector<dev::ChainItem*> ptrs;
+    for (DevBuf& b : g->itemBufs) ptrs.push_back(b.as<dev::ChainItem>());
+    const hipError_t err = enqueueGroups(groups, ptrs, sliceSweeps);
     const hipError_t endErr = hipStreamEndCapture(stream_, &g->graph);
     hipCheck(err, "annealSlice capture");
     hipCheck(endErr, "hipStreamEndCapture");

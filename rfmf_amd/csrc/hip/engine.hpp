@@ -277,9 +277,17 @@ private:
         int threads = dev::kThreads;
         dev::LdsPlan plan{};
         int grid = 1;
+        int occupancy = 1;   // resident workgroups per CU
         bool hbm = false;
     };
-    std::vector<LaunchGroup> planGroups(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs);
+    // `slice`: sweeps per launch (~0: to completion).
+    std::vector<LaunchGroup> planGroups(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs,
+                                        uint64_t slice);
+    void finishGroup(LaunchGroup& g, int64_t maxN, int maxP, const LaunchKnobs& knobs);
+    // Enqueues the groups of one slice (items[i]: group i's device item array)
+    // on the launch lanes; returns the first error.
+    hipError_t enqueueGroups(const std::vector<LaunchGroup>& groups, const std::vector<dev::ChainItem*>& items,
+                             uint64_t slice);
     static uint64_t tuneKey(int bits, int mode, int64_t maxN, int64_t chains, int maxP, bool hbm);
     bool fitsLds(int64_t maxN, int bits, int mode, int maxP, int threads, bool hbm = false) const;
     std::unordered_map<uint64_t, int> tuned_;
@@ -288,8 +296,19 @@ private:
     int device_;
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;
+    // Launch lanes: the groups of one slice run concurrently, lane 0 being
+    // stream_ and the others forked from it and joined back (fork_/join_).
+    // Three lanes: with the side stream that makes the four hardware queues
+    // of a process, so no lane shares a queue (and its order) with another.
+    static constexpr int kLanes = 3;
+    static constexpr size_t kMaxGroups = 128;
+    // Size classes only for launches of at least this many sweeps.
+    static constexpr uint64_t kClassMinSweeps = 4096;
+    hipStream_t lanes_[kLanes - 1] = {};
+    hipEvent_t fork_ = nullptr;
+    hipEvent_t join_[kLanes - 1] = {};
     hipDeviceProp_t props_{};
-    DevBuf sched_;   // 2 x u32 work counters, zero between launches
+    DevBuf sched_;   // kMaxGroups pairs of u32 work counters, zero between launches
     std::unordered_map<int32_t, std::unique_ptr<DevGraph>> graphs_;
     // Item arrays of enqueued slices: two slots used in turn, each a device
     // buffer, its pinned staging copy and the event after the launches that

@@ -80,6 +80,7 @@ def main() -> int:
     ap.add_argument("--threads", type=int, default=0, help="threads per chain (0: autotune)")
     ap.add_argument("--qps", type=float, default=20.0, help="serve: Poisson arrival rate")
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
+    ap.add_argument("--slice", type=int, default=500, help="serve: sweeps per serving slice")
     ap.add_argument("--graphs", type=int, default=256, help="batch: number of graphs")
     args = ap.parse_args()
 
@@ -357,7 +358,7 @@ def bench_serve(args) -> int:
             ctx.batch_run([(gid, j)])
             ctx.release_graph(gid)
             svc.append(time.perf_counter() - t)
-        with R.Server(ctx, hbm_budget_bytes=64 << 30, slice_sweeps=500) as srv:
+        with R.Server(ctx, hbm_budget_bytes=64 << 30, slice_sweeps=args.slice) as srv:
             t0 = time.perf_counter()
             ids = []
             for i, j in enumerate(served):

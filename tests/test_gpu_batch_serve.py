@@ -73,8 +73,11 @@ def test_batch_mixed_graphs_equal_one_shot(graphs):
     assert np.array_equal(j.end_spins, end) and np.array_equal(j.best_spins, best)
 
 
-@pytest.mark.parametrize("slice_sweeps,use_graph", [(1, False), (37, False), (50, True), (999, True)])
+@pytest.mark.parametrize("slice_sweeps,use_graph", [(1, False), (37, False), (50, True), (999, True),
+                                                     (5000, True)])
 def test_slicing_and_graph_replay_invariance(graphs, slice_sweeps, use_graph):
+    # 5000-sweep slices are long enough for size classes: the captured graph
+    # then forks the classes onto the concurrent launch lanes.
     print(f"[batch] slice={slice_sweeps} hipGraph={use_graph}", flush=True)
     jobs = _mixed_jobs(graphs)
     with R.Context(0) as ctx:

@@ -255,3 +255,41 @@ def test_flips_ladder_halves_to_one_and_splits_sweeps_evenly():
         with pytest.raises(R.RfmfError) as e:
             R.flips_ladder(bad, 10)
         assert e.value.status == status
+
+
+# C struct name -> ctypes mirror used by the Python package.
+_MIRRORS = {"rfmf_csr": "Csr", "rfmf_read_stats": "ReadStats", "rfmf_beta_tables": "BetaTables",
+            "rfmf_launch_cfg": "LaunchCfg", "rfmf_job": "Job", "rfmf_tuning": "Tuning",
+            "rfmf_flips_tune": "FlipsTune", "rfmf_flips_report": "FlipsReport",
+            "rfmf_server_stats": "ServerStats"}
+
+
+def test_header_is_c_and_struct_layouts_match_ctypes(tmp_path):
+    # The boundary is a plain C header: it compiles as C11 on its own, and
+    # every struct's size and field offsets equal the ctypes mirror's.
+    import shutil
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "rfmf.h"', 'int main(void) {']
+    for cname, pyname in _MIRRORS.items():
+        mirror = getattr(_capi, pyname)
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in mirror._fields_:
+            lines.append(f'    printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run([cc, "-std=c11", "-Wall", "-Werror", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        cname, what, val = line.split()
+        got[(cname, what)] = int(val)
+    for cname, pyname in _MIRRORS.items():
+        mirror = getattr(_capi, pyname)
+        assert got[(cname, "size")] == C.sizeof(mirror), cname
+        for field, _ in mirror._fields_:
+            assert got[(cname, field)] == getattr(mirror, field).offset, f"{cname}.{field}"

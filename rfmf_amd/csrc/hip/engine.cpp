@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <unordered_set>
 
 #include "../host/max_cut.hpp"
 
@@ -461,11 +462,24 @@ float Context::runSlice(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
     return waitSlice(h);
 }
 
+uint64_t Context::groupKey(int bits, int mode, bool hbm, int64_t topN, bool split) {
+    int cls = 0;   // size class: the largest power of two (>= 2^11) below topN
+    if (split)
+        for (int64_t edge = int64_t{1} << 11; edge < topN; edge <<= 1) ++cls;
+    return (static_cast<uint64_t>(hbm) << 40) | (static_cast<uint64_t>(bits) << 32) |
+           (static_cast<uint64_t>(mode) << 24) | static_cast<uint64_t>(cls);
+}
+
 void Context::finishGroup(LaunchGroup& g, int64_t maxN, int maxP, const LaunchKnobs& knobs) {
     // Tile, LDS plan, score arithmetic and grid for one launch group.
-    g.threads = knobs.threads > 0
-                    ? knobs.threads
-                    : chooseThreads(g.bits, g.mode, maxN, static_cast<int64_t>(g.items.size()), maxP, g.hbm);
+    int tuned = 0;
+    for (const auto& kv : knobs.groupThreads)
+        if (kv.first == g.key) tuned = kv.second;
+    g.maxP = maxP;
+    g.threads = knobs.threads > 0 ? knobs.threads
+                : tuned > 0      ? tuned
+                                 : chooseThreads(g.bits, g.mode, maxN, static_cast<int64_t>(g.items.size()), maxP,
+                                                 g.hbm);
     if (!dev::validThreads(g.threads))
         throw Error(Status::InvalidArg, "threads per chain must be 128, 256, 512 or 1024");
     g.plan = dev::makeLdsPlan(maxN, g.bits, maxP, g.threads, g.mode, g.hbm);
@@ -548,6 +562,7 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
             c.hbm = b.hbm;
             c.bits = b.bits;
             c.mode = b.mode;
+            c.key = groupKey(b.bits, b.mode, b.hbm, top, split);
             c.items.assign(b.items.begin() + static_cast<std::ptrdiff_t>(at),
                            b.items.begin() + static_cast<std::ptrdiff_t>(stop));
             finishGroup(c, top, maxP[gi], knobs);
@@ -723,64 +738,82 @@ float Context::waitSlice(SliceHandle& h) {
     return ms;
 }
 
-int Context::autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* spentMs, int keyP,
-                      uint64_t budget) {
-    // Shape of the launch: widest word and score, largest n and P, unfinished chains.
-    // A candidate tile must fit the LDS-resident chains' plan and, if any
-    // chain keeps its state in HBM, the HBM plan.
-    int bits = 8, mode = 0, maxP = 1;
-    int64_t maxN = 0, maxNLds = -1, chains = 0;
-    bool hbm = false;
-    uint64_t minLeft = ~uint64_t{0};
-    for (const ResidentJob* j : jobs)
-        for (size_t c = 0; c < j->items.size(); ++c) {
-            if (j->stepsDone[c] >= j->totalSteps) continue;
-            bits = std::max(bits, j->bits);
-            mode = std::max(mode, j->mode);
-            maxN = std::max<int64_t>(maxN, j->graph->n);
-            maxP = std::max(maxP, j->maxP);
-            if (stateInHbm(j->graph->n, j->bits, j->mode, j->maxP, knobs))
-                hbm = true;
-            else
-                maxNLds = std::max<int64_t>(maxNLds, j->graph->n);
-            ++chains;
-            minLeft = std::min(minLeft, j->totalSteps - j->stepsDone[c]);
-        }
-    if (chains == 0) return 0;
-    const uint64_t key = tuneKey(bits, mode, maxN, chains, keyP > 0 ? keyP : maxP, hbm);
-    const auto hit = tuned_.find(key);
+void Context::autotune(const std::vector<ResidentJob*>& jobs, LaunchKnobs* knobs, float* spentMs, int keyP,
+                       uint64_t budget) {
+    // The groups the launch will run, planned without a forced tile.
+    LaunchKnobs plain = *knobs;
+    plain.threads = 0;
+    plain.groupThreads.clear();
+    // The launches that follow cover `budget` sweeps at most, in slices of
+    // knobs->sliceSweeps: plan with that length, so the size classes match.
+    uint64_t slice = knobs->sliceSweeps > 0 ? static_cast<uint64_t>(knobs->sliceSweeps) : ~uint64_t{0};
+    if (budget > 0) slice = std::min(slice, budget);
+    const std::vector<LaunchGroup> groups = planGroups(jobs, plain, slice);
+    knobs->groupThreads.clear();
     last_.tuned = true;
-    if (hit != tuned_.end()) return hit->second;
-    // Each candidate runs one short slice of the live run: geometry does not
-    // change any chain's arithmetic, so the trial sweeps are part of the
-    // result (and their time is part of the caller's total), and the trials
-    // together use at most a twenty-fifth of the sweeps.  Candidates go from
-    // the smallest tile up; once one is a quarter slower than the best so far
-    // the larger ones are not tried.
-    const uint64_t span = budget > 0 ? std::min(budget, minLeft) : minLeft;
-    const uint64_t trial = std::max<uint64_t>(1, std::min<uint64_t>(128, span / 100));
-    int best = 0;
-    float bestPerSweep = 0.0f;
-    for (int i = 0; i < 4; ++i) {
-        const int t = dev::kThreadChoices[i];
-        if (maxNLds >= 0 && !fitsLds(maxNLds, bits, mode, maxP, t)) continue;
-        if (hbm && !fitsLds(0, bits, mode, maxP, t, true)) continue;
-        LaunchKnobs k = knobs;
-        k.threads = t;
-        k.sliceSweeps = static_cast<int64_t>(trial);
-        const float ms = runSlice(jobs, k);
-        if (spentMs) *spentMs += ms;
-        const float per = ms / static_cast<float>(trial);
-        last_.trialMsPerSweep[i] = per;
-        if (best == 0 || per < bestPerSweep) {
-            best = t;
-            bestPerSweep = per;
-        } else if (per > 1.25f * bestPerSweep) {
-            break;
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const LaunchGroup& g = groups[gi];
+        // The group's jobs: those whose unfinished chains it holds (a job's
+        // chains share one graph, so they fall in one group).
+        std::vector<ResidentJob*> sub;
+        int64_t maxN = 0;
+        uint64_t minLeft = ~uint64_t{0};
+        std::unordered_set<const uint32_t*> held;
+        for (const dev::ChainItem& it : g.items) held.insert(it.words);
+        for (ResidentJob* j : jobs) {
+            for (size_t c = 0; c < j->items.size(); ++c) {
+                if (j->stepsDone[c] >= j->totalSteps) continue;
+                if (held.count(j->items[c].words)) {
+                    sub.push_back(j);
+                    maxN = std::max<int64_t>(maxN, j->graph->n);
+                    for (size_t d = 0; d < j->items.size(); ++d)
+                        if (j->stepsDone[d] < j->totalSteps)
+                            minLeft = std::min(minLeft, j->totalSteps - j->stepsDone[d]);
+                }
+                break;
+            }
         }
+        if (sub.empty()) continue;
+        const uint64_t key = g.key * 0x9E3779B97F4A7C15ull ^
+                             tuneKey(g.bits, 0, maxN, static_cast<int64_t>(g.items.size()), keyP > 0 ? keyP : g.maxP,
+                                     g.hbm);
+        const auto hit = tuned_.find(key);
+        if (hit != tuned_.end()) {
+            knobs->groupThreads.emplace_back(g.key, hit->second);
+            continue;
+        }
+        // Each candidate runs one short slice of the group's live chains:
+        // geometry does not change any chain's arithmetic, so the trial
+        // sweeps are part of the result (and their time is part of the
+        // caller's total), and a group's trials together use at most a
+        // twenty-fifth of its sweeps.  Candidates go from the smallest tile
+        // up; once one is a quarter slower than the best so far the larger
+        // ones are not tried.
+        const uint64_t span = budget > 0 ? std::min(budget, minLeft) : minLeft;
+        const uint64_t trial = std::max<uint64_t>(1, std::min<uint64_t>(128, span / 100));
+        int best = 0;
+        float bestPerSweep = 0.0f;
+        for (int i = 0; i < 4; ++i) {
+            const int t = dev::kThreadChoices[i];
+            if (!fitsLds(g.hbm ? 0 : maxN, g.bits, g.mode, g.maxP, t, g.hbm)) continue;
+            LaunchKnobs k = plain;
+            k.threads = t;
+            k.sliceSweeps = static_cast<int64_t>(trial);
+            const float ms = runSlice(sub, k);
+            if (spentMs) *spentMs += ms;
+            const float per = ms / static_cast<float>(trial);
+            if (gi == 0) last_.trialMsPerSweep[i] = per;
+            if (best == 0 || per < bestPerSweep) {
+                best = t;
+                bestPerSweep = per;
+            } else if (per > 1.25f * bestPerSweep) {
+                break;
+            }
+        }
+        if (best == 0) continue;
+        tuned_[key] = best;
+        knobs->groupThreads.emplace_back(g.key, best);
     }
-    tuned_[key] = best;
-    return best;
 }
 
 float Context::runToCompletion(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs) {
@@ -821,7 +854,7 @@ float Context::runUntil(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
             const uint64_t end = std::min(j0->hPhaseEnd[ph], until);
             if (end <= at) continue;
             LaunchKnobs k = knobsIn;
-            k.threads = autotune(jobs, k, &total, static_cast<int>(j0->hPhaseP[ph]), end - at);
+            autotune(jobs, &k, &total, static_cast<int>(j0->hPhaseP[ph]), end - at);
             while (lowest(at) && at < end) {
                 k.sliceSweeps = sliceTo(at, end);
                 total += runSlice(jobs, k);
@@ -829,7 +862,7 @@ float Context::runUntil(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
         }
         return total;
     }
-    if (knobs.autotune && knobs.threads == 0) knobs.threads = autotune(jobs, knobs, &total);
+    if (knobs.autotune && knobs.threads == 0) autotune(jobs, &knobs, &total);
     if (knobs.hipGraph && knobs.sliceSweeps > 0 && until == ~uint64_t{0}) {
         // Every chain starts together here, so ceil(longest / slice) replays
         // of one captured slice finish them all.

@@ -162,6 +162,9 @@ struct LaunchKnobs {
     bool autotune = false;     // time the candidates on the run's first slices
     int scoreTable = 0;        // 8-bit words: 0 table when it costs no resident chains, 1 always, -1 never
     int stateInHbm = 0;        // 0: only chains whose state exceeds LDS keep it in HBM; 1: every chain
+    // Tiles the autotuner chose per launch group (Context::groupKey); a group
+    // not listed uses `threads`, else the cached choice or the heuristic.
+    std::vector<std::pair<uint64_t, int>> groupThreads;
 };
 
 // What the last launch of a context used (for reports).
@@ -244,14 +247,16 @@ public:
     };
     std::unique_ptr<GraphedSlices> captureSlices(const std::vector<ResidentJob*>& jobs, uint64_t sliceSweeps,
                                                  const LaunchKnobs& knobs);
-    // Autotuner: times each thread-count candidate on one short slice of the
-    // live run (output-invariant), caches and returns the fastest; the trial
-    // launches' kernel ms are added to *spentMs.  `keyP` (0: the jobs'
-    // largest P) and `budget` (0: the shortest remaining run) narrow the
-    // tuning to one phase: the cache key uses keyP and the trials together
-    // stay within a twenty-fifth of the budget.
-    int autotune(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs, float* spentMs, int keyP = 0,
-                 uint64_t budget = 0);
+    // Autotuner: plans the launch, then for each launch group (kernel variant
+    // and size class) times each thread-count candidate on one short slice of
+    // that group's own chains (output-invariant), caches the fastest and
+    // records it in knobs->groupThreads; the trial launches' kernel ms are
+    // added to *spentMs.  `keyP` (0: the group's largest P) and `budget` (0:
+    // the shortest remaining run) narrow the tuning to one phase: the cache
+    // key uses keyP and each group's trials stay within a twenty-fifth of
+    // the budget.
+    void autotune(const std::vector<ResidentJob*>& jobs, LaunchKnobs* knobs, float* spentMs, int keyP = 0,
+                  uint64_t budget = 0);
     // Replays the captured slice `times` times; returns kernel-side ms.
     float replaySlices(GraphedSlices& g, int times);
     // Copies a finished job's results to the spec's outputs.
@@ -278,8 +283,13 @@ private:
         dev::LdsPlan plan{};
         int grid = 1;
         int occupancy = 1;   // resident workgroups per CU
+        int maxP = 1;
+        uint64_t key = 0;    // groupKey: variant and size class
         bool hbm = false;
     };
+    // Identity of a launch group across plans: word bits, score mode before
+    // the table choice, state in HBM, and the size class of its largest n.
+    static uint64_t groupKey(int bits, int mode, bool hbm, int64_t topN, bool split);
     // `slice`: sweeps per launch (~0: to completion).
     std::vector<LaunchGroup> planGroups(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobs,
                                         uint64_t slice);

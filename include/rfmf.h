@@ -21,7 +21,7 @@ extern "C" {
 typedef enum rfmf_status {
     RFMF_OK = 0,
     RFMF_ERR_INVALID_ARG = 1,   /* sizes, null pointers, t0 <= 0, P out of range */
-    RFMF_ERR_UNSUPPORTED = 2,   /* max weighted degree >= 2^30, P > 4096 (group tables beyond LDS) */
+    RFMF_ERR_UNSUPPORTED = 2,   /* max weighted degree >= 2^62, P > 4096 (group tables beyond LDS) */
     RFMF_ERR_HIP = 3,           /* HIP runtime error (message has the detail) */
     RFMF_ERR_NO_DEVICE = 4,
     RFMF_ERR_OUT_OF_MEMORY = 5,
@@ -255,7 +255,9 @@ rfmf_status rfmf_run_advance(rfmf_run* run, int64_t sweeps, const rfmf_tuning* t
 rfmf_status rfmf_run_collect(rfmf_run* run);
 void rfmf_run_destroy(rfmf_run* run);
 
-/* HBM bytes a job of this shape keeps resident (CSR + chain state + tables). */
+/* HBM bytes a job of this shape keeps resident (CSR + chain state + tables),
+ * for state words of up to 32 bits (max weighted degree below 2^30; the
+ * server counts the 64-bit words of heavier graphs itself). */
 int64_t rfmf_footprint_bytes(int64_t n, int64_t m2, int64_t chains, int64_t beta_runs,
                              int64_t n_phases);
 

@@ -76,6 +76,13 @@ template <> struct WordFmt<32> {
     static constexpr int32_t kBias = 1 << 30;
     static constexpr int64_t kMaxB = (1 << 30) - 1;
 };
+// Weighted degrees of 2^30 and beyond (the reference's 64-bit S,
+// main.cu:51): one 64-bit word per vertex.
+template <> struct WordFmt<64> {
+    static constexpr uint64_t kSpin = 1ull << 63, kField = (1ull << 63) - 1;
+    static constexpr int64_t kBias = int64_t{1} << 62;
+    static constexpr int64_t kMaxB = (int64_t{1} << 62) - 1;
+};
 
 // Bytes of a chain's word array: n words rounded up to whole groups of 32
 // vertices (the unit the initial-state kernel writes), a multiple of 16 B.

@@ -166,6 +166,20 @@ __device__ __forceinline__ void atomicAddAt(uint32_t* p, uint32_t v) {
         atomicAdd(p, v);
 }
 
+template <bool kGlobal>
+__device__ __forceinline__ void atomicAddAt(uint64_t* p, uint64_t v) {
+    if constexpr (kGlobal)
+        __hip_atomic_fetch_add(globalIn(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
+}
+
+// A vertex's state word and its signed delta, by word width.
+template <int kBits>
+using WordOf = std::conditional_t<kBits == 64, uint64_t, uint32_t>;
+template <int kBits>
+using DeltaOf = std::conditional_t<kBits == 64, int64_t, int32_t>;
+
 // 8 consecutive words of a vertex block, as loaded from LDS (or HBM).
 template <int kBits> struct Block8 {
     static constexpr int kDw = kBits / 4;   // dwords holding 8 words
@@ -179,14 +193,18 @@ template <int kBits> struct Block8 {
             const u32x4 q = loadAt<kGlobal>(reinterpret_cast<const u32x4*>(wl) + b);
             d[0] = q.x; d[1] = q.y; d[2] = q.z; d[3] = q.w;
         } else {
-            const u32x4 q0 = loadAt<kGlobal>(reinterpret_cast<const u32x4*>(wl) + 2 * b);
-            const u32x4 q1 = loadAt<kGlobal>(reinterpret_cast<const u32x4*>(wl) + 2 * b + 1);
-            d[0] = q0.x; d[1] = q0.y; d[2] = q0.z; d[3] = q0.w;
-            d[4] = q1.x; d[5] = q1.y; d[6] = q1.z; d[7] = q1.w;
+            constexpr int kQ = kBits / 16;   // u32x4 loads per block
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const u32x4 v = loadAt<kGlobal>(reinterpret_cast<const u32x4*>(wl) + kQ * b + q);
+                d[4 * q] = v.x; d[4 * q + 1] = v.y; d[4 * q + 2] = v.z; d[4 * q + 3] = v.w;
+            }
         }
     }
-    __device__ __forceinline__ uint32_t word(int j) const {
-        if constexpr (kBits == 32) {
+    __device__ __forceinline__ WordOf<kBits> word(int j) const {
+        if constexpr (kBits == 64) {
+            return static_cast<uint64_t>(d[2 * j]) | (static_cast<uint64_t>(d[2 * j + 1]) << 32);
+        } else if constexpr (kBits == 32) {
             return d[j];
         } else {
             constexpr int per = 32 / kBits;
@@ -194,9 +212,9 @@ template <int kBits> struct Block8 {
         }
     }
     // Biased delta field of word j.
-    __device__ __forceinline__ uint32_t field(int j) const {
-        if constexpr (kBits == 32) {
-            return d[j] & WordFmt<32>::kField;
+    __device__ __forceinline__ WordOf<kBits> field(int j) const {
+        if constexpr (kBits >= 32) {
+            return word(j) & WordFmt<kBits>::kField;
         } else {
             constexpr int per = 32 / kBits;
             return (d[j / per] >> (kBits * (j % per))) & WordFmt<kBits>::kField;
@@ -212,8 +230,10 @@ template <int kBits> struct Block8 {
 };
 
 template <int kBits, bool kGlobal = false>
-__device__ __forceinline__ uint32_t ldsWord(const uint32_t* wl, int x) {
-    if constexpr (kBits == 32) {
+__device__ __forceinline__ WordOf<kBits> ldsWord(const uint32_t* wl, int x) {
+    if constexpr (kBits == 64) {
+        return loadAt<kGlobal>(reinterpret_cast<const uint64_t*>(wl) + x);
+    } else if constexpr (kBits == 32) {
         return loadAt<kGlobal>(wl + x);
     } else {
         constexpr int per = 32 / kBits;
@@ -224,10 +244,14 @@ __device__ __forceinline__ uint32_t ldsWord(const uint32_t* wl, int x) {
 // Adds d (mod 2^kBits, applied without carry or borrow because the word's
 // old and new values are both valid words) to word x.
 template <int kBits, bool kGlobal = false>
-__device__ __forceinline__ void ldsAddWord(uint32_t* wl, int x, uint32_t d) {
-    constexpr int per = 32 / kBits;
-    const int sh = kBits * (x % per);
-    atomicAddAt<kGlobal>(&wl[x / per], d << sh);
+__device__ __forceinline__ void ldsAddWord(uint32_t* wl, int x, WordOf<kBits> d) {
+    if constexpr (kBits == 64) {
+        atomicAddAt<kGlobal>(reinterpret_cast<uint64_t*>(wl) + x, d);
+    } else {
+        constexpr int per = 32 / kBits;
+        const int sh = kBits * (x % per);
+        atomicAddAt<kGlobal>(&wl[x / per], d << sh);
+    }
 }
 
 __device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -407,6 +431,38 @@ __device__ __forceinline__ int32_t waveSum32(int32_t x) {
     return a + b;
 }
 
+// Sum over aligned groups of L lanes (L a power of two up to 64), left in
+// every lane of the group.
+__device__ __forceinline__ int32_t laneGroupSum(int32_t x, int L, int lane) {
+    if (L > 1) x += laneXor<1>(x, lane);
+    if (L > 2) x += laneXor<2>(x, lane);
+    if (L > 4) x += laneXor<4>(x, lane);
+    if (L > 8) x += laneXor<8>(x, lane);
+    if (L > 16) { int32_t a, b; swapPair<16>(x, a, b); x = a + b; }
+    if (L > 32) { int32_t a, b; swapPair<32>(x, a, b); x = a + b; }
+    return x;
+}
+template <int kDist>
+__device__ __forceinline__ int64_t laneXor64(int64_t x, int lane) {
+    return join64(laneXor<kDist>(lo32(x), lane), laneXor<kDist>(hi32(x), lane));
+}
+template <int kDist>
+__device__ __forceinline__ int64_t swapSum64(int64_t x) {
+    int32_t la, lb, ha, hb;
+    swapPair<kDist>(lo32(x), la, lb);
+    swapPair<kDist>(hi32(x), ha, hb);
+    return join64(la, ha) + join64(lb, hb);
+}
+__device__ __forceinline__ int64_t laneGroupSum(int64_t x, int L, int lane) {
+    if (L > 1) x += laneXor64<1>(x, lane);
+    if (L > 2) x += laneXor64<2>(x, lane);
+    if (L > 4) x += laneXor64<4>(x, lane);
+    if (L > 8) x += laneXor64<8>(x, lane);
+    if (L > 16) x = swapSum64<16>(x);
+    if (L > 32) x = swapSum64<32>(x);
+    return x;
+}
+
 __device__ __forceinline__ int64_t waveSum64(int64_t x) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
@@ -428,9 +484,12 @@ __device__ __forceinline__ int32_t scoreOf(int32_t beta, uint32_t field, uint32_
         return static_cast<int32_t>(static_cast<uint32_t>(beta) * field + base);
 }
 
-// Exact int64 score: beta * field + (LUT - bias*beta) + fine.
-__device__ __forceinline__ int64_t scoreWide(int32_t beta, uint32_t field, int64_t lutBiased, uint32_t fine) {
-    return static_cast<int64_t>(beta) * static_cast<int64_t>(field) + lutBiased + static_cast<int64_t>(fine);
+// int64 score: beta * field + (LUT - bias*beta) + fine, all mod 2^64 (the
+// field is delta + bias): the exact score whenever it fits int64, as the
+// reference's int64 arithmetic gives it (src/anneal_kernels.cu:350-378).
+__device__ __forceinline__ int64_t scoreWide(int32_t beta, uint64_t field, int64_t lutBiased, uint32_t fine) {
+    return static_cast<int64_t>(static_cast<uint64_t>(static_cast<int64_t>(beta)) * field +
+                                static_cast<uint64_t>(lutBiased) + fine);
 }
 
 // Occupancy floor per variant: 6 waves per SIMD (at most 80 VGPRs) keeps 12
@@ -458,7 +517,10 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     using Score = std::conditional_t<kWide, int64_t, int32_t>;
     constexpr Score kNoScore = kWide ? static_cast<Score>(INT64_MIN) : static_cast<Score>(INT32_MIN);
     using F = WordFmt<kBits>;
+    using Word = WordOf<kBits>;
+    using Delta = DeltaOf<kBits>;
     static_assert(!kHbm || !kTab, "HBM-resident state runs the LUT variants");
+    static_assert(kBits < 64 || kWide, "64-bit words score in int64");
     extern __shared__ __align__(16) unsigned char lds[];
     uint32_t* const ldsWords = reinterpret_cast<uint32_t*>(lds + plan.wordsOff);
     uint32_t* const ldsBest = reinterpret_cast<uint32_t*>(lds + plan.bestOff);
@@ -466,7 +528,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     uint32_t* partId = reinterpret_cast<uint32_t*>(lds + plan.partIdOff);   // int64 scores only
     int32_t* pk = reinterpret_cast<int32_t*>(lds + plan.pickOff);
     int32_t* ps = reinterpret_cast<int32_t*>(lds + plan.spinOff);
-    int32_t* pd = reinterpret_cast<int32_t*>(lds + plan.deltaOff);
+    Delta* pd = reinterpret_cast<Delta*>(lds + plan.deltaOff);
     // Static LDS, so the table's address folds into the reads' offset field:
     // 16 x u32 (int32 scores) or 16 x int64 (int64 scores).
     __shared__ __align__(16) uint32_t lutRaw[32];
@@ -486,7 +548,9 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
         const int t = threadIdx.x;
         if (t < 16) {
             if constexpr (kWide)
-                lutw[t] = static_cast<int64_t>(kGumbelCoarse[t]) - static_cast<int64_t>(F::kBias) * b;
+                lutw[t] = static_cast<int64_t>(static_cast<uint64_t>(static_cast<int64_t>(kGumbelCoarse[t])) -
+                                               static_cast<uint64_t>(F::kBias) *
+                                                   static_cast<uint64_t>(static_cast<int64_t>(b)));
             else
                 lutb[t] = static_cast<uint32_t>(kGumbelCoarse[t]) -
                           static_cast<uint32_t>(F::kBias) * static_cast<uint32_t>(b);
@@ -793,12 +857,13 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                         partId[g] = 0u;
                     }
                 }
-                int32_t pick = -1, sp = 0, dl = 0;
+                int32_t pick = -1, sp = 0;
+                Delta dl = 0;
                 if (key != 0ull) {
                     pick = static_cast<int32_t>(~(kWide ? kid : static_cast<uint32_t>(key)));
-                    const uint32_t w = ldsWord<kBits, kHbm>(wl, pick);
+                    const Word w = ldsWord<kBits, kHbm>(wl, pick);
                     sp = (w & F::kSpin) ? 1 : -1;
-                    dl = static_cast<int32_t>(w & F::kField) - F::kBias;
+                    dl = static_cast<Delta>(w & F::kField) - static_cast<Delta>(F::kBias);
                 }
                 pk[g] = pick;
                 ps[g] = sp;
@@ -839,7 +904,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                     const int g = pass * perPass + (tid >> lgL);
                     int32_t b = -1, sb = 0;
                     if (g < P) { b = pk[g]; sb = ps[g]; }
-                    int32_t own = 0;   // sum over adjacent winners x of 2 w s_b s_x
+                    Delta own = 0;   // sum over adjacent winners x of 2 w s_b s_x
                     if (b >= 0) {
                         const uint32_t lo = offsets[b];
                         const uint32_t hi = offsets[b + 1];
@@ -858,38 +923,34 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                             const int gx1 = pow2 ? (e1.x & (P - 1)) : (e1.x % P);
                             const int32_t pk0 = pk[gx0];
                             const int32_t pk1 = pk[gx1];
-                            const uint32_t wd0 = ldsWord<kBits, kHbm>(wl, e0.x);
-                            const uint32_t wd1 = ldsWord<kBits, kHbm>(wl, e1.x);
-                            auto apply = [&](int x, int32_t w2, int gx, int32_t pkx, uint32_t wd) {
+                            const Word wd0 = ldsWord<kBits, kHbm>(wl, e0.x);
+                            const Word wd1 = ldsWord<kBits, kHbm>(wl, e1.x);
+                            auto apply = [&](int x, Delta w2, int gx, int32_t pkx, Word wd) {
                                 if (pkx == x) {
                                     // x flips too: its coupling enters b's own
                                     // update, and, for an earlier group, the cut
                                     // delta as sequential group order sees it.
-                                    const int32_t t = w2 * sb * ps[gx];
+                                    const Delta t = w2 * sb * ps[gx];
                                     own += t;
                                     if (gx < g) dsum -= t;
                                 } else {
                                     // delta(x) += s_x * (-2 w s_b); x's spin is stable.
-                                    const int32_t d = (wd & F::kSpin) ? -w2 * sb : w2 * sb;
-                                    ldsAddWord<kBits, kHbm>(wl, x, static_cast<uint32_t>(d));
+                                    const Delta d = (wd & F::kSpin) ? -w2 * sb : w2 * sb;
+                                    ldsAddWord<kBits, kHbm>(wl, x, static_cast<Word>(d));
                                 }
                             };
-                            apply(e0.x, 2 * e0.y, gx0, pk0, wd0);
-                            if (two) apply(e1.x, 2 * e1.y, gx1, pk1, wd1);
+                            apply(e0.x, Delta{2} * e0.y, gx0, pk0, wd0);
+                            if (two) apply(e1.x, Delta{2} * e1.y, gx1, pk1, wd1);
                         }
                     }
                     // Sum over the winner's L lanes (an aligned lane group).
-                    if (L > 1) own += laneXor<1>(own, lane);
-                    if (L > 2) own += laneXor<2>(own, lane);
-                    if (L > 4) own += laneXor<4>(own, lane);
-                    if (L > 8) own += laneXor<8>(own, lane);
-                    if (L > 16) { int32_t a, b; swapPair<16>(own, a, b); own = a + b; }
-                    if (L > 32) { int32_t a, b; swapPair<32>(own, a, b); own = a + b; }
+                    own = laneGroupSum(own, L, lane);
                     if (b >= 0 && sub == 0) {
                         dsum += pd[g];
                         // delta(b) -> -delta(b) + own, spin bit toggled: one add.
-                        const uint32_t toggle = sb > 0 ? 0u - F::kSpin : F::kSpin;
-                        ldsAddWord<kBits, kHbm>(wl, b, static_cast<uint32_t>(own) - 2u * static_cast<uint32_t>(pd[g]) + toggle);
+                        const Word toggle = sb > 0 ? Word{0} - F::kSpin : F::kSpin;
+                        ldsAddWord<kBits, kHbm>(wl, b,
+                                                static_cast<Word>(own) - Word{2} * static_cast<Word>(pd[g]) + toggle);
                     }
                 }
             }
@@ -956,6 +1017,8 @@ __global__ __launch_bounds__(256) void initWords(const InitItem* __restrict__ it
     const int2* edges = reinterpret_cast<const int2*>(it.edges);
     for (int grp = blockIdx.x * blockDim.x + threadIdx.x; grp < nGroups;
          grp += gridDim.x * blockDim.x) {
+        using Word = WordOf<kBits>;
+        using Delta = DeltaOf<kBits>;
         uint32_t bits = 0;
         uint32_t packed[kBits];
 #pragma unroll
@@ -963,19 +1026,22 @@ __global__ __launch_bounds__(256) void initWords(const InitItem* __restrict__ it
 #pragma unroll 4
         for (int k = 0; k < 32; ++k) {
             const int v = 32 * grp + k;
-            uint32_t word = 0u;
+            Word word = 0u;
             if (v < n) {
-                int32_t S = 0;
+                Delta S = 0;
                 for (uint32_t i = it.offsets[v]; i < it.offsets[v + 1]; ++i) {
                     const int2 e = edges[i];
-                    S += e.y * static_cast<int32_t>(it.spins[e.x]);
+                    S += static_cast<Delta>(e.y) * static_cast<Delta>(it.spins[e.x]);
                 }
                 const bool up = it.spins[v] > 0;
-                const int32_t delta = up ? S : -S;
-                word = (up ? F::kSpin : 0u) | static_cast<uint32_t>(delta + F::kBias);
+                const Delta delta = up ? S : -S;
+                word = (up ? F::kSpin : Word{0}) | static_cast<Word>(delta + static_cast<Delta>(F::kBias));
                 bits |= (up ? 1u : 0u) << k;
             }
-            if constexpr (kBits == 32) {
+            if constexpr (kBits == 64) {
+                packed[2 * k] = static_cast<uint32_t>(word);
+                packed[2 * k + 1] = static_cast<uint32_t>(word >> 32);
+            } else if constexpr (kBits == 32) {
                 packed[k] = word;
             } else {
                 constexpr int per = 32 / kBits;
@@ -1028,7 +1094,9 @@ static const void* kernelFor(int threads) {
 template <int kBits, bool kHbm>
 static const void* kernelForMode(int mode, int threads) {
     switch (mode) {
-        case dev::kScore32: return kernelFor<kBits, dev::kScore32, kHbm>(threads);
+        case dev::kScore32:   // int32 scores up to 32-bit words
+            if constexpr (kBits <= 32) return kernelFor<kBits, dev::kScore32, kHbm>(threads);
+            return nullptr;
         case dev::kScore24:   // 24-bit fields only (8/16-bit words)
             if constexpr (kBits <= 16) return kernelFor<kBits, dev::kScore24, kHbm>(threads);
             return nullptr;
@@ -1045,6 +1113,7 @@ static const void* kernelFor(int bits, int mode, int threads, bool hbm) {
         case 8:  return hbm ? kernelForMode<8, true>(mode, threads) : kernelForMode<8, false>(mode, threads);
         case 16: return hbm ? kernelForMode<16, true>(mode, threads) : kernelForMode<16, false>(mode, threads);
         case 32: return hbm ? kernelForMode<32, true>(mode, threads) : kernelForMode<32, false>(mode, threads);
+        case 64: return hbm ? kernelForMode<64, true>(mode, threads) : kernelForMode<64, false>(mode, threads);
         default: return nullptr;
     }
 }
@@ -1081,6 +1150,7 @@ hipError_t initWords(int bits, const InitItem* items, int nItems, int maxN, hipS
         case 8:  hipLaunchKernelGGL(dev::initWords<8>, grid, dim3(256), 0, s, items, nItems); break;
         case 16: hipLaunchKernelGGL(dev::initWords<16>, grid, dim3(256), 0, s, items, nItems); break;
         case 32: hipLaunchKernelGGL(dev::initWords<32>, grid, dim3(256), 0, s, items, nItems); break;
+        case 64: hipLaunchKernelGGL(dev::initWords<64>, grid, dim3(256), 0, s, items, nItems); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

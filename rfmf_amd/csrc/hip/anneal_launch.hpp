@@ -80,7 +80,7 @@ inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads, int mo
     if (mode == kScore64) off += alignUp(static_cast<uint32_t>(slots) * 4u, 16);
     p.pickOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
     p.spinOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
-    p.deltaOff = off; off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
+    p.deltaOff = off; off += alignUp(static_cast<uint32_t>(maxP) * (bits == 64 ? 8u : 4u), 16);
     p.dpartOff = off; off += alignUp(static_cast<uint32_t>(waves) * 8u, 16);
     p.miscOff = off;  off += 16;
     p.total = off;

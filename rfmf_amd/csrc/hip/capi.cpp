@@ -39,6 +39,12 @@ namespace rfmf {
 void setLastError(const std::string& msg) { g_lastError = msg; }
 }  // namespace rfmf
 
+int64_t rfmf::footprintBytes(int64_t n, int64_t m2, int64_t chains, int64_t beta_runs, int64_t n_phases,
+                             int bits) {
+    return (n + 1) * 4 + m2 * 8 + chains * (rfmf::dev::wordBytes(n, bits) + rfmf::dev::bestBytes(n) + 32) +
+           beta_runs * 12 + n_phases * 12 + chains * 8;
+}
+
 extern "C" {
 
 const char* rfmf_status_string(rfmf_status s) {
@@ -396,9 +402,7 @@ rfmf_status rfmf_ctx_copy_probe(rfmf_ctx* ctx, int64_t bytes, int32_t iters, flo
 }
 
 int64_t rfmf_footprint_bytes(int64_t n, int64_t m2, int64_t chains, int64_t beta_runs, int64_t n_phases) {
-    const int bits = 32;   // conservative: the widest state word
-    return (n + 1) * 4 + m2 * 8 + chains * (rfmf::dev::wordBytes(n, bits) + rfmf::dev::bestBytes(n) + 32) +
-           beta_runs * 12 + n_phases * 12 + chains * 8;
+    return rfmf::footprintBytes(n, m2, chains, beta_runs, n_phases, 32);   // the widest word below B = 2^30
 }
 
 namespace {

@@ -18,6 +18,9 @@ namespace rfmf {
 // Sets the calling thread's rfmf_last_error() message.
 void setLastError(const std::string& msg);
 
+// HBM bytes a job keeps resident with state words of `bits` bits.
+int64_t footprintBytes(int64_t n, int64_t m2, int64_t chains, int64_t beta_runs, int64_t n_phases, int bits);
+
 // Runs f and maps what it throws to a status code and the thread's
 // rfmf_last_error() message (cleared on success).
 template <class F>

@@ -66,6 +66,7 @@ int wordBitsFor(int64_t B) {
     if (B <= dev::WordFmt<8>::kMaxB) return 8;
     if (B <= dev::WordFmt<16>::kMaxB) return 16;
     if (B <= dev::WordFmt<32>::kMaxB) return 32;
+    if (B <= dev::WordFmt<64>::kMaxB) return 64;
     return 0;
 }
 
@@ -328,13 +329,13 @@ HostJob Context::prepareJob(const JobSpec& spec, const HostGraph& g) const {
 
     hj.bits = wordBitsFor(g.B);
     if (hj.bits == 0)
-        throw Error(Status::Unsupported, "max weighted degree >= 2^30 is not supported");
+        throw Error(Status::Unsupported, "max weighted degree >= 2^62 is not supported");
     // Score arithmetic: int32 when the gate holds (src/host_common.cpp:296-312,
     // verdict at the largest beta), with 24-bit multiplies when |beta| < 2^23
     // and the field is below 2^23 (8/16-bit words); exact int64 otherwise.
     const int64_t gateB = std::max<int64_t>(spec.gateB, g.B);
-    if (!scoreSafeInt32ForBeta(betaMax, gateB))
-        hj.mode = dev::kScore64;
+    if (hj.bits == 64 || !scoreSafeInt32ForBeta(betaMax, gateB))
+        hj.mode = dev::kScore64;   // 64-bit words always score in int64
     else if (betaMax < (int64_t{1} << 23) && hj.bits <= 16)
         hj.mode = dev::kScore24;
     else
@@ -516,11 +517,11 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
     // Group unfinished chains by kernel variant (word bits, score mode, state
     // in LDS or HBM); items sorted by descending n so the persistent
     // workgroups take the longest chains first.
-    constexpr int kVariants = 2 * 3 * dev::kScoreModes;
+    constexpr int kVariants = 2 * 4 * dev::kScoreModes;
     std::vector<LaunchGroup> base(kVariants);
     std::vector<int> maxP(kVariants, 1);
     auto gidx = [](bool hbm, int bits, int mode) {
-        return ((hbm ? 3 : 0) + (bits == 8 ? 0 : bits == 16 ? 1 : 2)) * dev::kScoreModes + mode;
+        return ((hbm ? 4 : 0) + (bits == 8 ? 0 : bits == 16 ? 1 : bits == 32 ? 2 : 3)) * dev::kScoreModes + mode;
     };
     for (ResidentJob* j : jobs) {
         const bool hbm = stateInHbm(j->graph->n, j->bits, j->mode, j->maxP, knobs);

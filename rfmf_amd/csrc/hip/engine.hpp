@@ -337,6 +337,6 @@ private:
     std::mutex mu_;
 };
 
-int wordBitsFor(int64_t B);   // 8 / 16 / 32, or 0 if unsupported
+int wordBitsFor(int64_t B);   // 8 / 16 / 32 / 64, or 0 if unsupported
 
 }  // namespace rfmf

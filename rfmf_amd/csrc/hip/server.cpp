@@ -260,7 +260,9 @@ rfmf_status rfmf_server_submit(rfmf_server* srv, const rfmf_csr* g, const rfmf_j
     s.bestSpins = job->best_spins;
     s.bestVals = job->best_vals;
     s.finalVals = job->final_vals;
-    r->footprint = rfmf_footprint_bytes(g->n, g->m2, job->chains, job->beta.n_runs, job->n_phases);
+    const int bits = rfmf::wordBitsFor(rfmf_max_weighted_degree(g));
+    r->footprint = rfmf::footprintBytes(g->n, g->m2, job->chains, job->beta.n_runs, job->n_phases,
+                                        bits > 32 ? bits : 32);
     auto reject = [&](rfmf_status st, const std::string& why) {
         rfmf::setLastError(why);
         std::lock_guard<std::mutex> lk(srv->mu);

@@ -241,3 +241,23 @@ def test_state_beyond_lds_u32_words():
     phases, iters = [32, 1], [40, 40]
     _hbm_case("cycle-45k-u32", csr, [5, 6], R.build_linear_beta_table(0.0, 0.001, sum(iters), 2), phases, iters,
               32)
+
+
+# ---------------------------------------------------------- 64-bit words
+# A max weighted degree of 2^30 or more needs 64-bit state words (the
+# reference's 64-bit S, main.cu:51); they score in int64.
+
+def test_words_u64_in_lds():
+    csr = _cycle_with_chords(500, 1 << 29, seed=15)
+    assert R.max_weighted_degree(csr) >= 2**30
+    phases, iters = [16, 4, 1], [60, 60, 60]
+    beta = R.build_linear_beta_table(0.0, 2e-6, sum(iters), 3)
+    _case("u64-words", csr, [7, 8, 9], beta, phases, iters)
+
+
+def test_words_u64_in_hbm():
+    csr = _cycle_with_chords(30000, 1 << 29, seed=16)
+    assert R.max_weighted_degree(csr) >= 2**30
+    phases, iters = [64, 1], [30, 30]
+    _hbm_case("cycle-30k-u64", csr, [10, 11], R.build_linear_beta_table(0.0, 2e-6, sum(iters), 2), phases, iters,
+              64)

@@ -51,7 +51,7 @@ struct alignas(16) ChainItem {
     int32_t n;                 // vertices
     int32_t nRuns;
     int32_t nPhases;
-    int32_t pad;
+    int32_t deltaMax;          // bound on |delta| (max weighted degree, capped at 2^31 - 1)
 };
 static_assert(sizeof(ChainItem) == 112, "ChainItem layout");
 

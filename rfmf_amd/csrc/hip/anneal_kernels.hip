@@ -55,6 +55,9 @@ __constant__ int32_t kGumbelCoarse[16] = {
     -1303301, -903532, -648633, -438929, -249398, -68827,  109563,  290966,
     480289,   683080,  906615,  1161749, 1466888, 1858654, 2430921, 3617486};
 
+// Largest |Gumbel| in Q20: coarse[15] + 127 (src/philox.cuh:46-48).
+constexpr uint64_t kGumbelMaxMag = 3617613u;
+
 constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
 constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
 constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
@@ -495,24 +498,26 @@ __device__ __forceinline__ int64_t scoreWide(int32_t beta, uint64_t field, int64
 // Occupancy floor per variant: 6 waves per SIMD (at most 80 VGPRs) keeps 12
 // two-wave chains resident per CU in large batches; 32-bit words and
 // HBM-resident state take 5 and int64 scores 4, where fewer registers would
-// spill.  The score-table variant runs launches whose chains are all
-// resident anyway, so its registers go to the schedule (floor 1).
-template <int kBits, int kMode, bool kHbm>
+// spill.  The score table with 8- and 16-wave tiles serves launches of one
+// or two chains per CU, bound by each sweep's latency, so its registers go
+// to the schedule (floor 1).
+template <int kBits, int kMode, int NT, bool kHbm>
 constexpr int kSliceWavesPerEu =
-    kMode == kScore64 ? 4 : kMode == kScoreTab ? 1 : (kBits == 32 || kHbm) ? 5 : 6;
+    kMode == kScoreTab && NT >= 512 ? 1 : kMode == kScore64 ? 4 : (kBits == 32 || kHbm) ? 5 : 6;
 
 // kHbm: the chain's words and best bits stay in HBM and are updated in place
 // (graphs whose state does not fit LDS).  FLIP's global atomics complete in
 // L2, so after the sweep's last barrier each wave invalidates its L1 (an
 // agent-scope acquire) before the next sweep reads the words.
 template <int kBits, int kMode, int NT, bool kHbm>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kSliceWavesPerEu<kBits, kMode, kHbm>)))
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kSliceWavesPerEu<kBits, kMode, NT, kHbm>)))
 void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __restrict__ sched, LdsPlan plan,
                  uint64_t sliceSweeps) {
     constexpr int NW = NT / 64;
     constexpr bool kTab = kMode == kScoreTab;
     constexpr bool kMul24 = kMode == kScore24 || kTab;
     constexpr bool kWide = kMode == kScore64;
+    constexpr bool kPackable = !kWide && NT <= 256;   // packed per-lane argmax keys
     static_assert(!kTab || kBits == 8, "the score table needs 8-bit words (field < 128)");
     using Score = std::conditional_t<kWide, int64_t, int32_t>;
     constexpr Score kNoScore = kWide ? static_cast<Score>(INT64_MIN) : static_cast<Score>(INT32_MIN);
@@ -643,6 +648,11 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
         // fewer blocks than threads).  The others skip the per-group
         // reduction: their partial entries stay zero (empty).
         const int busyWaves = nBlk >= NT ? NW : (nBlk + 63) >> 6;
+        // Blocks per lane (K) and the key bits that number them.
+        const int lanesK = (nBlk + NT - 1) / NT;
+        const int lanesBits = lanesK <= 1 ? 0 : 32 - __clz(lanesK - 1);
+        const int lanesFull = nFull / NT;   // full blocks every lane scores
+        const uint64_t deltaMax = static_cast<uint64_t>(uni(it.deltaMax));
         // Best-spin snapshot: the spins after an improving sweep, taken before
         // the next sweep's flips (and at the end of the slice).  Each thread
         // packs its own blocks; nothing else touches bl8 until the copy-out.
@@ -677,6 +687,18 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
             const PhiloxSweep phx = philoxSweep(sLo, sHi, k0, k1);
             const bool pow2 = (P & (P - 1)) == 0;
             const bool laneReduce = pow2 && P <= kMaxLaneReduceP && ((8 * NT) % P) == 0;
+            // Packed per-lane keys when every score of the sweep satisfies
+            // |score| <= |beta| * deltaMax + kGumbelMaxMag < 2^(31 - lanesBits),
+            // and each lane has enough blocks (3 or more full ones) for the
+            // cheaper loop to repay the per-sweep decode.  Only the 2- and
+            // 4-wave tiles carry this path: the larger ones serve launches of
+            // one or two chains per CU, whose chains rarely have that many
+            // blocks per lane, and whose sweeps are latency-bound.
+            const uint64_t absBeta = static_cast<uint64_t>(beta < 0 ? -static_cast<int64_t>(beta) : beta);
+            const int packShift = kPackable && lanesFull >= 3 &&
+                                          absBeta * deltaMax + kGumbelMaxMag < (uint64_t{1} << (31 - lanesBits))
+                                      ? lanesBits
+                                      : -1;
 
             // ---------------- SELECT ----------------
             // Per-wave partial of a group: a packed (score, ~id) key for int32
@@ -692,12 +714,97 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
             // int64 scores with LDS slots take two passes (the max score, then
             // the smallest id holding it); this flag marks the second.
             bool idPass = false;
+            // One vertex's score (int32 variants): the true score, or for the
+            // table variant the true score plus bias * beta.
+            auto score32 = [&](const Block8<kBits>& blk, int j, uint32_t r) {
+                if constexpr (kTab)
+                    return __mul24(beta, static_cast<int32_t>(blk.field(j))) + gtab[r & 0x7FFu];
+                else
+                    return scoreOf<kMul24>(beta, static_cast<uint32_t>(blk.field(j)), lutb[r & 15u], (r >> 4) & 0x7Fu);
+            };
+            // Per-lane argmax with packed keys: key = score * 2^packShift +
+            // (K - 1 - k) for the lane's k-th block, so one max keeps the
+            // best score and, among equal scores, the earlier block (the
+            // smaller vertex).  Exact while |score| < 2^(31 - packShift),
+            // which the per-sweep bound guarantees.
+            auto scorePackedLanes = [&](Score (&accS)[8], int32_t (&accV)[8]) {
+                int32_t acc[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[j] = INT32_MIN;
+                const int cb = kPackable ? packShift : 0;
+                const uint32_t biasBeta = kTab ? static_cast<uint32_t>(F::kBias) * static_cast<uint32_t>(beta) : 0u;
+                const uint32_t c0 = static_cast<uint32_t>(lanesK - 1) - (biasBeta << cb);
+                auto scoreBlockPacked = [&](int b, uint32_t c, bool tail) {
+                    Block8<kBits> blk;
+                    blk.template load<kHbm>(wl, b);
+                    uint32_t o[4];
+                    philoxBlock(phx, static_cast<uint32_t>(b), o);
+                    // The byte offsets of both fields of an output word come
+                    // from one shifted copy: (w << 2) & mask and its high half
+                    // & mask (a word-select operand).  The copy is opaque to
+                    // the compiler, which would otherwise fold it back into a
+                    // shift and a mask per field.
+                    uint32_t o4[4];
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) {
+                        o4[w] = o[w] << 2;
+                        asm("" : "+v"(o4[w]));
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        if (tail && 8 * b + j >= n) continue;
+                        uint32_t t;
+                        if constexpr (kTab) {   // table entry r & 0x7FF
+                            const uint32_t off = ((j & 1) ? (o4[j >> 1] >> 16) : o4[j >> 1]) & 0x1FFCu;
+                            const int32_t g = *reinterpret_cast<const int32_t*>(
+                                reinterpret_cast<const unsigned char*>(gtab) + off);
+                            t = static_cast<uint32_t>(__mul24(beta, static_cast<int32_t>(blk.field(j))) + g);
+                        } else {                // LUT entry r & 15
+                            const uint32_t off = ((j & 1) ? (o4[j >> 1] >> 16) : o4[j >> 1]) & 0x3Cu;
+                            const uint32_t lut = *reinterpret_cast<const uint32_t*>(
+                                reinterpret_cast<const unsigned char*>(lutb) + off);
+                            const uint32_t r = ((j & 1) ? (o[j >> 1] >> 16) : o[j >> 1]);
+                            t = static_cast<uint32_t>(
+                                scoreOf<kMul24>(beta, static_cast<uint32_t>(blk.field(j)), lut, (r >> 4) & 0x7Fu));
+                        }
+                        acc[j] = max(acc[j], static_cast<int32_t>((t << cb) + c));
+                    }
+                };
+                // K - 1 - k lives in a VGPR: a key is then one v_lshl_add_u32
+                // (shift in an SGPR, block number in a VGPR), since one VALU
+                // instruction reads at most one SGPR.  Every lane runs the
+                // nFull / NT full rounds (a uniform loop), the lanes below
+                // nFull mod NT one more block, and one lane the partial block.
+                uint32_t c = c0;
+                asm volatile("" : "+v"(c));
+                int b = tid;
+                for (int k = 0; k < lanesFull; ++k, b += NT, --c) scoreBlockPacked(b, c, false);
+                if (b < nFull) scoreBlockPacked(b, c, false);
+                if (nFull < nBlk && tid == (nFull % NT)) scoreBlockPacked(nFull, c, true);
+                // Every lane scored at least one full block, so no slot is
+                // empty.  Decode: score = key >> shift (plus bias * beta for
+                // the table), block = tid + (K - 1 - low bits) * NT.
+                const uint32_t mask = (1u << cb) - 1u;
+                const int32_t bTop = tid + (lanesK - 1) * NT;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    accS[j] = static_cast<Score>(static_cast<uint32_t>(acc[j] >> cb) + biasBeta);
+                    accV[j] = bTop - static_cast<int32_t>(static_cast<uint32_t>(acc[j]) & mask) * NT;
+                }
+            };
             auto select = [&](auto laneTag) {
                 constexpr bool kLane = decltype(laneTag)::value;
                 Score accS[8];
                 int32_t accV[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) { accS[j] = kNoScore; accV[j] = -1; }
+                bool scored = false;
+                if constexpr (kLane && kPackable) {
+                    if (packShift >= 0) {
+                        scorePackedLanes(accS, accV);
+                        scored = true;
+                    }
+                }
                 auto scoreBlock = [&](int b, bool tail) {
                     Block8<kBits> blk;
                     blk.template load<kHbm>(wl, b);
@@ -708,14 +815,11 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                         const int v = 8 * b + j;
                         if (tail && v >= n) continue;
                         const uint32_t r = ((j & 1) ? (o[j >> 1] >> 16) : o[j >> 1]);
-                        const uint32_t fine = (r >> 4) & 0x7Fu;
                         Score score;
                         if constexpr (kWide)
-                            score = scoreWide(beta, blk.field(j), lutw[r & 15u], fine);
-                        else if constexpr (kTab)
-                            score = __mul24(beta, static_cast<int32_t>(blk.field(j))) + gtab[r & 0x7FFu];
+                            score = scoreWide(beta, blk.field(j), lutw[r & 15u], (r >> 4) & 0x7Fu);
                         else
-                            score = scoreOf<kMul24>(beta, blk.field(j), lutb[r & 15u], fine);
+                            score = score32(blk, j, r);
                         if constexpr (kLane) {
                             // slot j keeps the block index; the vertex is 8*b + j
                             if (score > accS[j]) { accS[j] = score; accV[j] = b; }
@@ -734,8 +838,10 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                         }
                     }
                 };
-                for (int b = tid; b < nFull; b += NT) scoreBlock(b, false);
-                if (nFull < nBlk && tid == (nFull % NT)) scoreBlock(nFull, true);
+                if (!scored) {
+                    for (int b = tid; b < nFull; b += NT) scoreBlock(b, false);
+                    if (nFull < nBlk && tid == (nFull % NT)) scoreBlock(nFull, true);
+                }
 
                 if constexpr (kLane) {
                     if (wave >= busyWaves) return;

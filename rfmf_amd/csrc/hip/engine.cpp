@@ -454,6 +454,7 @@ std::unique_ptr<ResidentJob> Context::placeJob(const HostJob& hj, const DevGraph
         it.key = hj.seeds[c];
         it.stepEnd = hj.totalSteps;
         it.n = static_cast<int32_t>(n);
+        it.deltaMax = static_cast<int32_t>(std::min<int64_t>(g->B, INT32_MAX));
     }
     return job;
 }

@@ -261,3 +261,71 @@ def test_words_u64_in_hbm():
     phases, iters = [64, 1], [30, 30]
     _hbm_case("cycle-30k-u64", csr, [10, 11], R.build_linear_beta_table(0.0, 2e-6, sum(iters), 2), phases, iters,
               64)
+
+
+def _weighted_torus(rows, cols, wmax, seed):
+    rng = np.random.default_rng(seed)
+    idx = np.arange(rows * cols).reshape(rows, cols)
+    u = np.concatenate([idx.ravel(), idx.ravel()])
+    v = np.concatenate([np.roll(idx, -1, axis=1).ravel(), np.roll(idx, -1, axis=0).ravel()])
+    w = rng.integers(1, wmax + 1, u.size) * np.where(rng.random(u.size) < 0.5, -1, 1)
+    return R.graph_from_edges(u, v, w)
+
+
+@pytest.mark.parametrize("threads,score_table,hbm", [(128, 1, False), (256, 1, False), (256, -1, False),
+                                                     (128, 0, True)])
+def test_packed_lane_keys_switch_off_mid_run(threads, score_table, hbm):
+    """A lane's running argmax packs (score, block) into one int32 key while
+    |beta| * B + max|Gumbel| < 2^(31 - key bits), the key bits numbering the
+    lane's blocks.  With B near 60 and 20000 vertices the linear schedule
+    crosses that bound mid-run, so every chain runs sweeps of both forms."""
+    csr = _weighted_torus(100, 200, 15, seed=21)
+    B = R.max_weighted_degree(csr)
+    assert 40 < B <= 63
+    lane_blocks = -(-((csr.n + 7) // 8) // threads)
+    key_bits = (lane_blocks - 1).bit_length()
+    bound = 2 ** (31 - key_bits)
+    lo, hi = round(0.1 * 2**20), round(3.0 * 2**20)
+    assert lo * B + 3617613 < bound <= hi * B + 3617613
+    seeds = [1, 2, 3]
+    phases, iters = [128, 16, 1], [60, 60, 60]
+    beta = R.build_linear_beta_table(0.1, 3.0, sum(iters), len(seeds))
+    print(f"[gpu-parity] packed keys: threads={threads} score_table={score_table} hbm={hbm} B={B}", flush=True)
+    init = np.stack([R.generate_random_spins(s, csr.n) for s in seeds])
+    solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+    job = R.Job(csr, init, solver, beta, phases, iters)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ctx.batch_run([(gid, job)], threads=threads, score_table=score_table, state_in_hbm=hbm)
+        geo = ctx.last_launch()
+    assert geo["threads_per_chain"] == threads and geo["word_bits"] == 8 and geo["state_in_hbm"] == hbm, geo
+    assert geo["score_mode"] == ("mul24-table" if score_table == 1 else "int32-mul24"), geo
+    end, best, cv, bv = run_oracle_chains(csr, init, solver, beta, phases, iters)
+    assert np.array_equal(job.end_spins, end), "final spins differ"
+    assert np.array_equal(job.best_spins, best), "best spins differ"
+    assert np.array_equal(job.final_vals, cv) and np.array_equal(job.best_vals, bv), "cuts differ"
+
+
+@pytest.mark.parametrize("threads,score_table", [(128, 1), (256, -1)])
+def test_packed_lane_keys_ragged_tail(threads, score_table):
+    """Packed per-lane keys on a chain whose last block is partial (n mod 8 =
+    7) and whose full blocks do not divide evenly over the lanes."""
+    csr = _cycle_with_chords(9999, 10, seed=23)
+    assert csr.n % 8 == 7 and R.max_weighted_degree(csr) <= 63
+    assert (csr.n // 8) // threads >= 3 and (csr.n // 8) % threads != 0
+    seeds = [4, 5]
+    phases, iters = [64, 8, 1], [50, 50, 50]
+    beta = R.build_linear_beta_table(0.1, 3.0, sum(iters), len(seeds))
+    print(f"[gpu-parity] packed keys ragged: threads={threads} score_table={score_table}", flush=True)
+    init = np.stack([R.generate_random_spins(s, csr.n) for s in seeds])
+    solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+    job = R.Job(csr, init, solver, beta, phases, iters)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ctx.batch_run([(gid, job)], threads=threads, score_table=score_table)
+        geo = ctx.last_launch()
+    assert geo["threads_per_chain"] == threads and geo["word_bits"] == 8, geo
+    end, best, cv, bv = run_oracle_chains(csr, init, solver, beta, phases, iters)
+    assert np.array_equal(job.end_spins, end), "final spins differ"
+    assert np.array_equal(job.best_spins, best), "best spins differ"
+    assert np.array_equal(job.final_vals, cv) and np.array_equal(job.best_vals, bv), "cuts differ"

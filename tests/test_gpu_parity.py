@@ -329,3 +329,28 @@ def test_packed_lane_keys_ragged_tail(threads, score_table):
     assert np.array_equal(job.end_spins, end), "final spins differ"
     assert np.array_equal(job.best_spins, best), "best spins differ"
     assert np.array_equal(job.final_vals, cv) and np.array_equal(job.best_vals, bv), "cuts differ"
+
+
+def test_packed_lane_keys_u32_words():
+    """Packed per-lane keys over full 32-bit multiplies (32-bit words, int32
+    scores without the 24-bit multiplier): B = 43893 puts the key bound at
+    beta = 0.0028, so a linear schedule to 0.006 runs both forms."""
+    csr = _cycle_with_chords(10003, 9000, seed=31)
+    B = R.max_weighted_degree(csr)
+    assert B > 16383
+    seeds = [6, 7]
+    phases, iters = [64, 4, 1], [40, 40, 40]
+    beta = R.build_linear_beta_table(0.0, 0.006, sum(iters), len(seeds))
+    print(f"[gpu-parity] packed keys u32: n={csr.n} B={B}", flush=True)
+    init = np.stack([R.generate_random_spins(s, csr.n) for s in seeds])
+    solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+    job = R.Job(csr, init, solver, beta, phases, iters)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ctx.batch_run([(gid, job)], threads=128)
+        geo = ctx.last_launch()
+    assert geo["threads_per_chain"] == 128 and geo["word_bits"] == 32 and geo["score_mode"] == "int32", geo
+    end, best, cv, bv = run_oracle_chains(csr, init, solver, beta, phases, iters)
+    assert np.array_equal(job.end_spins, end), "final spins differ"
+    assert np.array_equal(job.best_spins, best), "best spins differ"
+    assert np.array_equal(job.final_vals, cv) and np.array_equal(job.best_vals, bv), "cuts differ"

@@ -354,3 +354,28 @@ def test_packed_lane_keys_u32_words():
     assert np.array_equal(job.end_spins, end), "final spins differ"
     assert np.array_equal(job.best_spins, best), "best spins differ"
     assert np.array_equal(job.final_vals, cv) and np.array_equal(job.best_vals, bv), "cuts differ"
+
+
+def test_packed_lane_keys_u16_words():
+    """Packed per-lane keys over 16-bit words (24-bit multiplies, LUT
+    scores): B = 176 puts the key bound at beta = 0.71, inside the linear
+    schedule 0.1 -> 2.0, and n mod 8 = 5 leaves a partial last block."""
+    csr = _cycle_with_chords(12005, 40, seed=37)
+    B = R.max_weighted_degree(csr)
+    assert 63 < B <= 16383
+    seeds = [8, 9]
+    phases, iters = [128, 16, 1], [50, 50, 50]
+    beta = R.build_linear_beta_table(0.1, 2.0, sum(iters), len(seeds))
+    print(f"[gpu-parity] packed keys u16: n={csr.n} B={B}", flush=True)
+    init = np.stack([R.generate_random_spins(s, csr.n) for s in seeds])
+    solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+    job = R.Job(csr, init, solver, beta, phases, iters)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ctx.batch_run([(gid, job)], threads=128)
+        geo = ctx.last_launch()
+    assert geo["threads_per_chain"] == 128 and geo["word_bits"] == 16 and geo["score_mode"] == "int32-mul24", geo
+    end, best, cv, bv = run_oracle_chains(csr, init, solver, beta, phases, iters)
+    assert np.array_equal(job.end_spins, end), "final spins differ"
+    assert np.array_equal(job.best_spins, best), "best spins differ"
+    assert np.array_equal(job.final_vals, cv) and np.array_equal(job.best_vals, bv), "cuts differ"

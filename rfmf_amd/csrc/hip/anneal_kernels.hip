@@ -25,9 +25,11 @@
 //   * Philox's block-independent half of rounds 1-3 is computed once per
 //     sweep on the scalar unit;
 //   * the per-group argmax is kept in registers (each lane's 8 vertex slots
-//     map to fixed groups when P divides 8*threads), combined across lanes by
-//     a reduce-scatter on DPP and v_permlane{16,32}_swap, and across waves
-//     through LDS: no atomics on the hot path;
+//     map to fixed groups when P divides 8*threads; in 2- and 4-wave tiles as
+//     one packed int32 key per slot, score and block number, kept with one
+//     v_max_i32), combined across lanes by a reduce-scatter on DPP and
+//     v_permlane{16,32}_swap, and across waves through LDS: no atomics on the
+//     hot path;
 //   * the winner map for corr[] is implicit: vertex x can only win group
 //     x mod P, so "x is a winner" is pick[x mod P] == x and no n-sized
 //     winnerGroup array exists;

@@ -379,3 +379,20 @@ def test_packed_lane_keys_u16_words():
     assert np.array_equal(job.end_spins, end), "final spins differ"
     assert np.array_equal(job.best_spins, best), "best spins differ"
     assert np.array_equal(job.final_vals, cv) and np.array_equal(job.best_vals, bv), "cuts differ"
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 9, 17, 63])
+def test_tiny_graphs(n):
+    """Fewer vertices than one block, one lane or one wave: a partial first
+    block, groups beyond n (P = 64 and 16 leave most groups empty), and the
+    ladder down to a single group."""
+    rng = np.random.default_rng(100 + n)
+    iu, ju = np.triu_indices(n, 1)
+    keep = rng.random(iu.size) < 0.6
+    keep[0] = True
+    u, v = iu[keep], ju[keep]
+    w = rng.integers(1, 4, u.size) * np.where(rng.random(u.size) < 0.5, -1, 1)
+    csr = R.graph_from_edges(u, v, w)
+    phases, iters = [64, 16, 4, 1], [25, 25, 25, 25]
+    beta = R.build_linear_beta_table(0.1, 3.0, sum(iters), 3)
+    _case(f"tiny n={csr.n}", csr, [11, 12, 13], beta, phases, iters)

@@ -260,6 +260,14 @@ __device__ __forceinline__ void ldsAddWord(uint32_t* wl, int x, WordOf<kBits> d)
 }
 
 __device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// x mod p for 0 <= x < 2^31 and p >= 2, with m = floor((2^32 - 1) / p): the
+// quotient estimate mulhi(x, m) is floor(x / p) or one less, since
+// x * (2^32 - p * m) < p * 2^32, so one correction makes the remainder exact.
+__device__ __forceinline__ int modBy(uint32_t x, uint32_t p, uint32_t m) {
+    const uint32_t r = x - __umulhi(x, m) * p;
+    return static_cast<int>(r >= p ? r - p : r);
+}
 __device__ __forceinline__ uint32_t uni(uint32_t x) {
     return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int32_t>(x)));
 }
@@ -502,20 +510,25 @@ __device__ __forceinline__ int64_t scoreWide(int32_t beta, uint64_t field, int64
 // HBM-resident state take 5 and int64 scores 4, where fewer registers would
 // spill.  The score table with 8- and 16-wave tiles serves launches of one
 // or two chains per CU, bound by each sweep's latency, so its registers go
-// to the schedule (floor 1).
-template <int kBits, int kMode, int NT, bool kHbm>
+// to the schedule (floor 1).  kRich: the score table's 2- and 4-wave tiles
+// with a floor of 5 (96 VGPRs), which the host picks when a sixth chain per
+// CU would not be resident anyway; at 80 VGPRs the compiler serializes the
+// scoring loop's table reads.
+template <int kBits, int kMode, int NT, bool kHbm, bool kRich>
 constexpr int kSliceWavesPerEu =
-    kMode == kScoreTab && NT >= 512 ? 1 : kMode == kScore64 ? 4 : (kBits == 32 || kHbm) ? 5 : 6;
+    kRich ? 5 : kMode == kScoreTab && NT >= 512 ? 1 : kMode == kScore64 ? 4 : (kBits == 32 || kHbm) ? 5 : 6;
 
 // kHbm: the chain's words and best bits stay in HBM and are updated in place
 // (graphs whose state does not fit LDS).  FLIP's global atomics complete in
 // L2, so after the sweep's last barrier each wave invalidates its L1 (an
 // agent-scope acquire) before the next sweep reads the words.
-template <int kBits, int kMode, int NT, bool kHbm>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kSliceWavesPerEu<kBits, kMode, NT, kHbm>)))
+template <int kBits, int kMode, int NT, bool kHbm, bool kRich = false>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kSliceWavesPerEu<kBits, kMode, NT, kHbm, kRich>)))
 void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __restrict__ sched, LdsPlan plan,
                  uint64_t sliceSweeps) {
     constexpr int NW = NT / 64;
+    constexpr int kLgNT = NT == 128 ? 7 : NT == 256 ? 8 : NT == 512 ? 9 : 10;
+    static_assert((1 << kLgNT) == NT, "threads per chain: 128, 256, 512 or 1024");
     constexpr bool kTab = kMode == kScoreTab;
     constexpr bool kMul24 = kMode == kScore24 || kTab;
     constexpr bool kWide = kMode == kScore64;
@@ -683,12 +696,20 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 nextEnd = static_cast<uint64_t>(uni(betaEnd[run + 1]));
                 nextVal = uni(betaVal[run + 1]);
             }
+            // The group map without integer division unless P is not a power
+            // of two (a mask, else a reciprocal), and the register-argmax
+            // condition (a power of two divides 8 * threads iff it does not
+            // exceed it).
+            const bool pow2 = (P & (P - 1)) == 0;
+            const bool laneReduce = pow2 && P <= kMaxLaneReduceP && P <= 8 * NT;
+            auto recipOfP = [&]() { return pow2 ? 0u : 0xFFFFFFFFu / static_cast<uint32_t>(P); };
+            auto groupOf = [&](int x, uint32_t pRecip) {
+                return pow2 ? (x & (P - 1)) : modBy(static_cast<uint32_t>(x), static_cast<uint32_t>(P), pRecip);
+            };
             const uint64_t s1 = step + 1;   // 1-based sweep id, the Philox counter
             const uint32_t sLo = static_cast<uint32_t>(s1);
             const uint32_t sHi = static_cast<uint32_t>(s1 >> 32);
             const PhiloxSweep phx = philoxSweep(sLo, sHi, k0, k1);
-            const bool pow2 = (P & (P - 1)) == 0;
-            const bool laneReduce = pow2 && P <= kMaxLaneReduceP && ((8 * NT) % P) == 0;
             // Packed per-lane keys when every score of the sweep satisfies
             // |score| <= |beta| * deltaMax + kGumbelMaxMag < 2^(31 - lanesBits),
             // and each lane has enough blocks (3 or more full ones) for the
@@ -796,6 +817,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
             };
             auto select = [&](auto laneTag) {
                 constexpr bool kLane = decltype(laneTag)::value;
+                const uint32_t pRecipSel = kLane ? 0u : recipOfP();   // slot path: groups by v mod P
                 Score accS[8];
                 int32_t accV[8];
 #pragma unroll
@@ -826,7 +848,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                             // slot j keeps the block index; the vertex is 8*b + j
                             if (score > accS[j]) { accS[j] = score; accV[j] = b; }
                         } else {
-                            const int g = pow2 ? (v & (P - 1)) : (v % P);
+                            const int g = groupOf(v, pRecipSel);
                             if constexpr (kWide) {
                                 if (!idPass)
                                     atomicMax(reinterpret_cast<unsigned long long*>(&part[g]),
@@ -996,14 +1018,14 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
             // ---------------- FLIP ----------------
             int64_t dsum = 0;
             {
-                int lgL = 0;
-                if (P < NT) {
-                    lgL = 31 - __clz(NT / P);
-                    lgL = lgL > 6 ? 6 : lgL;   // a winner's lanes stay inside one wave
-                }
-                const int L = 1 << lgL;
+                // 2^lgL = 2^floor(log2(threads / P)) lanes per winner (at most
+                // a wave), from ceil(log2 P) without a division.
+                const int ceilLgP = P <= 1 ? 0 : 32 - __clz(P - 1);
+                const int lgL = kLgNT - ceilLgP < 0 ? 0 : (kLgNT - ceilLgP > 6 ? 6 : kLgNT - ceilLgP);
                 const int perPass = NT >> lgL;
-                const int passes = (P + perPass - 1) / perPass;
+                const int passes = (P + perPass - 1) >> (kLgNT - lgL);
+                const uint32_t pRecip = recipOfP();
+                const int L = 1 << lgL;
                 const int sub = tid & (L - 1);
                 for (int pass = 0; pass < passes; ++pass) {
                     // Groups rise with the pass and the lane: a wave whose
@@ -1027,8 +1049,8 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                             const bool two = i1 < hi;
                             const i32x2 e0 = edges[i0];
                             const i32x2 e1 = two ? edges[i1] : i32x2{e0.x, 0};
-                            const int gx0 = pow2 ? (e0.x & (P - 1)) : (e0.x % P);
-                            const int gx1 = pow2 ? (e1.x & (P - 1)) : (e1.x % P);
+                            const int gx0 = groupOf(e0.x, pRecip);
+                            const int gx1 = groupOf(e1.x, pRecip);
                             const int32_t pk0 = pk[gx0];
                             const int32_t pk1 = pk[gx1];
                             const Word wd0 = ldsWord<kBits, kHbm>(wl, e0.x);
@@ -1189,7 +1211,11 @@ hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s) {
 // The compiled variants: word width x score mode x threads per chain x
 // state in LDS or HBM.
 template <int kBits, int kMode, bool kHbm>
-static const void* kernelFor(int threads) {
+static const void* kernelFor(int threads, bool rich) {
+    if constexpr (kMode == dev::kScoreTab && !kHbm) {
+        if (rich && threads == 128) return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 128, kHbm, true>);
+        if (rich && threads == 256) return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 256, kHbm, true>);
+    }
     switch (threads) {
         case 128: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 128, kHbm>);
         case 256: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 256, kHbm>);
@@ -1200,35 +1226,35 @@ static const void* kernelFor(int threads) {
 }
 
 template <int kBits, bool kHbm>
-static const void* kernelForMode(int mode, int threads) {
+static const void* kernelForMode(int mode, int threads, bool rich) {
     switch (mode) {
         case dev::kScore32:   // int32 scores up to 32-bit words
-            if constexpr (kBits <= 32) return kernelFor<kBits, dev::kScore32, kHbm>(threads);
+            if constexpr (kBits <= 32) return kernelFor<kBits, dev::kScore32, kHbm>(threads, false);
             return nullptr;
         case dev::kScore24:   // 24-bit fields only (8/16-bit words)
-            if constexpr (kBits <= 16) return kernelFor<kBits, dev::kScore24, kHbm>(threads);
+            if constexpr (kBits <= 16) return kernelFor<kBits, dev::kScore24, kHbm>(threads, false);
             return nullptr;
-        case dev::kScore64: return kernelFor<kBits, dev::kScore64, kHbm>(threads);
+        case dev::kScore64: return kernelFor<kBits, dev::kScore64, kHbm>(threads, false);
         case dev::kScoreTab:   // 8-bit words with LDS-resident state only
-            if constexpr (kBits == 8 && !kHbm) return kernelFor<kBits, dev::kScoreTab, kHbm>(threads);
+            if constexpr (kBits == 8 && !kHbm) return kernelFor<kBits, dev::kScoreTab, kHbm>(threads, rich);
             return nullptr;
         default: return nullptr;
     }
 }
 
-static const void* kernelFor(int bits, int mode, int threads, bool hbm) {
+static const void* kernelFor(int bits, int mode, int threads, bool hbm, bool rich) {
     switch (bits) {
-        case 8:  return hbm ? kernelForMode<8, true>(mode, threads) : kernelForMode<8, false>(mode, threads);
-        case 16: return hbm ? kernelForMode<16, true>(mode, threads) : kernelForMode<16, false>(mode, threads);
-        case 32: return hbm ? kernelForMode<32, true>(mode, threads) : kernelForMode<32, false>(mode, threads);
-        case 64: return hbm ? kernelForMode<64, true>(mode, threads) : kernelForMode<64, false>(mode, threads);
+        case 8:  return hbm ? kernelForMode<8, true>(mode, threads, rich) : kernelForMode<8, false>(mode, threads, rich);
+        case 16: return hbm ? kernelForMode<16, true>(mode, threads, rich) : kernelForMode<16, false>(mode, threads, rich);
+        case 32: return hbm ? kernelForMode<32, true>(mode, threads, rich) : kernelForMode<32, false>(mode, threads, rich);
+        case 64: return hbm ? kernelForMode<64, true>(mode, threads, rich) : kernelForMode<64, false>(mode, threads, rich);
         default: return nullptr;
     }
 }
 
-hipError_t anneal(int bits, int mode, int threads, bool hbm, const ChainItem* items, int nItems, uint32_t* sched,
-                  const LdsPlan& plan, int grid, uint64_t sliceSweeps, hipStream_t s) {
-    const void* kern = kernelFor(bits, mode, threads, hbm);
+hipError_t anneal(int bits, int mode, int threads, bool hbm, bool rich, const ChainItem* items, int nItems,
+                  uint32_t* sched, const LdsPlan& plan, int grid, uint64_t sliceSweeps, hipStream_t s) {
+    const void* kern = kernelFor(bits, mode, threads, hbm, rich);
     if (!kern) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        static_cast<int>(plan.total));
@@ -1238,8 +1264,8 @@ hipError_t anneal(int bits, int mode, int threads, bool hbm, const ChainItem* it
     return hipLaunchKernel(kern, dim3(grid), dim3(threads), args, plan.total, s);
 }
 
-int maxActiveBlocks(int bits, int mode, int threads, bool hbm, size_t lds) {
-    const void* kern = kernelFor(bits, mode, threads, hbm);
+int maxActiveBlocks(int bits, int mode, int threads, bool hbm, bool rich, size_t lds) {
+    const void* kern = kernelFor(bits, mode, threads, hbm, rich);
     if (!kern) return 0;
     if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) !=
         hipSuccess)

@@ -96,13 +96,15 @@ namespace launch {
 // Persistent slice kernel over `nItems` chain items, each advanced by at most
 // `sliceSweeps` sweeps, `threads` threads per workgroup; `sched` is a 2-word
 // device counter pair, zero before the launch and left zero after it.
-// `hbm`: the chains' state stays in HBM (no LDS copy).
-hipError_t anneal(int bits, int mode, int threads, bool hbm, const dev::ChainItem* items, int nItems,
+// `hbm`: the chains' state stays in HBM (no LDS copy).  `rich`: the score
+// table's 2- and 4-wave tiles built for 5 resident workgroups per CU instead
+// of 6 (more registers for the scoring loop's schedule); ignored elsewhere.
+hipError_t anneal(int bits, int mode, int threads, bool hbm, bool rich, const dev::ChainItem* items, int nItems,
                   uint32_t* sched, const dev::LdsPlan& plan, int grid, uint64_t sliceSweeps,
                   hipStream_t s);
 
 // Resident workgroups per CU for a variant at a given dynamic LDS size.
-int maxActiveBlocks(int bits, int mode, int threads, bool hbm, size_t lds);
+int maxActiveBlocks(int bits, int mode, int threads, bool hbm, bool rich, size_t lds);
 
 // Device-to-device copy of `bytes` (a multiple of 16) with a plain kernel.
 hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s);

@@ -209,12 +209,13 @@ const DevGraph* Context::graph(int32_t id) const {
     return it->second.get();
 }
 
-int Context::occupancy(int bits, int mode, int threads, size_t lds, bool hbm) {
-    const uint64_t key = (static_cast<uint64_t>(hbm) << 56) | (static_cast<uint64_t>(bits) << 48) |
+int Context::occupancy(int bits, int mode, int threads, size_t lds, bool hbm, bool rich) {
+    const uint64_t key = (static_cast<uint64_t>(rich) << 57) | (static_cast<uint64_t>(hbm) << 56) |
+                         (static_cast<uint64_t>(bits) << 48) |
                          (static_cast<uint64_t>(mode) << 44) | (static_cast<uint64_t>(threads) << 32) | lds;
     const auto it = occCache_.find(key);
     if (it != occCache_.end()) return it->second;
-    const int occ = launch::maxActiveBlocks(bits, mode, threads, hbm, lds);
+    const int occ = launch::maxActiveBlocks(bits, mode, threads, hbm, rich, lds);
     occCache_[key] = occ;
     return occ;
 }
@@ -509,6 +510,17 @@ void Context::finishGroup(LaunchGroup& g, int64_t maxN, int maxP, const LaunchKn
             }
         }
     }
+    // The score table's 2- and 4-wave tiles have a register-rich build (5
+    // workgroups per CU instead of 6): taken when it keeps every chain
+    // resident that the lean one would.
+    g.rich = false;
+    if (g.mode == dev::kScoreTab && g.threads <= 256) {
+        const int occRich = occupancy(8, dev::kScoreTab, g.threads, g.plan.total, false, true);
+        if (occRich > 0 && residentOf(occRich) >= residentOf(occ)) {
+            g.rich = true;
+            occ = occRich;
+        }
+    }
     g.occupancy = occ;
     g.grid = static_cast<int>(residentOf(occ));
 }
@@ -623,7 +635,7 @@ hipError_t Context::enqueueGroups(const std::vector<LaunchGroup>& groups,
     for (size_t i = 0; i < groups.size() && err == hipSuccess; ++i) {
         const LaunchGroup& g = groups[i];
         const size_t l = i % kLanes;
-        keep(launch::anneal(g.bits, g.mode, g.threads, g.hbm, items[i], static_cast<int>(g.items.size()),
+        keep(launch::anneal(g.bits, g.mode, g.threads, g.hbm, g.rich, items[i], static_cast<int>(g.items.size()),
                             sched_.as<uint32_t>() + 2 * i, g.plan, g.grid, slice,
                             l == 0 ? stream_ : lanes_[l - 1]));
     }
@@ -695,7 +707,7 @@ std::unique_ptr<Context::GraphedSlices> Context::captureSlices(const std::vector
         g->itemBufs.emplace_back(lg.items.size() * sizeof(dev::ChainItem));
         hipCheck(hipMemcpy(g->itemBufs.back().get(), lg.items.data(), g->itemBufs.back().bytes(),
                            hipMemcpyHostToDevice), "upload items");
-        (void)launch::maxActiveBlocks(lg.bits, lg.mode, lg.threads, lg.hbm, lg.plan.total);   // LDS attribute, outside the capture
+        (void)launch::maxActiveBlocks(lg.bits, lg.mode, lg.threads, lg.hbm, lg.rich, lg.plan.total);   // LDS attribute, outside the capture
     }
     hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");

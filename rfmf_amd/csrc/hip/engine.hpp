@@ -263,7 +263,7 @@ public:
     void collect(const ResidentJob& job, const JobSpec& spec, hipStream_t stream);
 
     // Resident workgroups per CU of a kernel variant (cached).
-    int occupancy(int bits, int mode, int threads, size_t lds, bool hbm = false);
+    int occupancy(int bits, int mode, int threads, size_t lds, bool hbm = false, bool rich = false);
     // Threads per chain for a launch shape: the autotuner's cached choice,
     // else a heuristic from chains per CU and n.
     int chooseThreads(int bits, int mode, int64_t maxN, int64_t chains, int maxP, bool hbm = false) const;
@@ -286,6 +286,7 @@ private:
         int maxP = 1;
         uint64_t key = 0;    // groupKey: variant and size class
         bool hbm = false;
+        bool rich = false;   // score table built for 5 workgroups per CU (launch::anneal)
     };
     // Identity of a launch group across plans: word bits, score mode before
     // the table choice, state in HBM, and the size class of its largest n.

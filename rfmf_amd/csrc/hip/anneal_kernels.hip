@@ -506,9 +506,9 @@ __device__ __forceinline__ int64_t scoreWide(int32_t beta, uint64_t field, int64
 }
 
 // Occupancy floor per variant: 6 waves per SIMD (at most 80 VGPRs) keeps 12
-// two-wave chains resident per CU in large batches; 32-bit words and
-// HBM-resident state take 5 and int64 scores 4, where fewer registers would
-// spill.  The score table with 8- and 16-wave tiles serves launches of one
+// two-wave chains resident per CU in large batches; 32-bit words take 5, and
+// int64 scores and HBM-resident state (graphs too large for LDS, a chain or
+// two per CU) 4, where fewer registers would spill.  The score table with 8- and 16-wave tiles serves launches of one
 // or two chains per CU, bound by each sweep's latency, so its registers go
 // to the schedule (floor 1).  kRich: the score table's 2- and 4-wave tiles
 // with a floor of 5 (96 VGPRs), which the host picks when a sixth chain per
@@ -516,7 +516,7 @@ __device__ __forceinline__ int64_t scoreWide(int32_t beta, uint64_t field, int64
 // scoring loop's table reads.
 template <int kBits, int kMode, int NT, bool kHbm, bool kRich>
 constexpr int kSliceWavesPerEu =
-    kRich ? 5 : kMode == kScoreTab && NT >= 512 ? 1 : kMode == kScore64 ? 4 : (kBits == 32 || kHbm) ? 5 : 6;
+    kRich ? 5 : kMode == kScoreTab && NT >= 512 ? 1 : (kMode == kScore64 || kHbm) ? 4 : kBits == 32 ? 5 : 6;
 
 // kHbm: the chain's words and best bits stay in HBM and are updated in place
 // (graphs whose state does not fit LDS).  FLIP's global atomics complete in

@@ -540,6 +540,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     using Word = WordOf<kBits>;
     using Delta = DeltaOf<kBits>;
     static_assert(!kHbm || !kTab, "HBM-resident state runs the LUT variants");
+    static_assert(!kRich || (kTab && NT <= 256), "register-rich builds: score table, 2- and 4-wave tiles");
     static_assert(kBits < 64 || kWide, "64-bit words score in int64");
     extern __shared__ __align__(16) unsigned char lds[];
     uint32_t* const ldsWords = reinterpret_cast<uint32_t*>(lds + plan.wordsOff);
@@ -772,6 +773,25 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                     for (int w = 0; w < 4; ++w) {
                         o4[w] = o[w] << 2;
                         asm("" : "+v"(o4[w]));
+                    }
+                    if constexpr (kRich) {
+                        // The register-rich build issues the block's eight
+                        // table reads together, ahead of every use (with few
+                        // waves per SIMD nothing else hides their latency).
+                        int32_t gv[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const uint32_t off = ((j & 1) ? (o4[j >> 1] >> 16) : o4[j >> 1]) & 0x1FFCu;
+                            gv[j] = *reinterpret_cast<const int32_t*>(reinterpret_cast<const unsigned char*>(gtab) + off);
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            if (tail && 8 * b + j >= n) continue;
+                            const uint32_t t = static_cast<uint32_t>(__mul24(beta, static_cast<int32_t>(blk.field(j))) + gv[j]);
+                            acc[j] = max(acc[j], static_cast<int32_t>((t << cb) + c));
+                        }
+                        return;
                     }
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {

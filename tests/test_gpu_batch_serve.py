@@ -255,3 +255,32 @@ def test_windows_with_per_phase_autotune_equal_one_shot(graphs):
         ms = ctx.batch_run_windows([(gid, job)], [50, 170], autotune=True)
     assert len(ms) == 2 and all(m > 0 for m in ms)
     same(job, one_shot(job), "windows")
+
+
+def test_score_table_lean_and_rich_builds():
+    """The score table's 4-wave tile has a lean build (6 workgroups per CU)
+    for launches with more chains than the register-rich build (5 per CU)
+    keeps resident, and the rich build otherwise: a chain's result is the
+    same in both, and the oracle's."""
+    csr = W.torus(40, 50, seed=3)   # n = 2000, 8-bit words
+    phases, iters = [128, 8, 1], [40, 40, 40]
+    with R.Context(0) as ctx:
+        cu = ctx.device_info()["cu"]
+        big = make_job(csr, list(range(1, 6 * cu + 1)), phases, iters)
+        small = make_job(csr, list(range(1, 49)), phases, iters)
+        gid = ctx.upload_graph(csr)
+        ctx.batch_run([(gid, big)], threads=256, score_table=1)
+        geo_big = ctx.last_launch()
+        ctx.batch_run([(gid, small)], threads=256, score_table=1)
+        geo_small = ctx.last_launch()
+    assert geo_big["score_mode"] == geo_small["score_mode"] == "mul24-table"
+    assert geo_big["wg_per_cu"] == 6 and geo_small["wg_per_cu"] == 5, (geo_big, geo_small)
+    for a, b in ((big.end_spins, small.end_spins), (big.best_spins, small.best_spins),
+                 (big.best_vals, small.best_vals), (big.final_vals, small.final_vals)):
+        assert np.array_equal(a[:48], b), "lean and rich builds differ"
+    init = small.init[:8]
+    solver = small.seeds[:8]
+    beta = R.build_linear_beta_table(0.1, 3.0, sum(iters), 8)
+    end, best, cv, bv = run_oracle_chains(csr, init, solver, beta, phases, iters)
+    assert np.array_equal(small.end_spins[:8], end) and np.array_equal(small.best_spins[:8], best)
+    assert np.array_equal(small.final_vals[:8], cv) and np.array_equal(small.best_vals[:8], bv)

@@ -510,10 +510,11 @@ __device__ __forceinline__ int64_t scoreWide(int32_t beta, uint64_t field, int64
 // int64 scores and HBM-resident state (graphs too large for LDS, a chain or
 // two per CU) 4, where fewer registers would spill.  The score table with 8- and 16-wave tiles serves launches of one
 // or two chains per CU, bound by each sweep's latency, so its registers go
-// to the schedule (floor 1).  kRich: the score table's 2- and 4-wave tiles
-// with a floor of 5 (96 VGPRs), which the host picks when a sixth chain per
-// CU would not be resident anyway; at 80 VGPRs the compiler serializes the
-// scoring loop's table reads.
+// to the schedule (floor 1).  kRich: the 2- and 4-wave tiles of the 24-bit
+// score variants (table and LUT, state in LDS) with a floor of 5 (96 VGPRs),
+// which the host picks when a sixth chain per CU would not be resident
+// anyway; at 80 VGPRs the compiler serializes the scoring loop's table
+// reads.
 template <int kBits, int kMode, int NT, bool kHbm, bool kRich>
 constexpr int kSliceWavesPerEu =
     kRich ? 5 : kMode == kScoreTab && NT >= 512 ? 1 : (kMode == kScore64 || kHbm) ? 4 : kBits == 32 ? 5 : 6;
@@ -540,7 +541,8 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     using Word = WordOf<kBits>;
     using Delta = DeltaOf<kBits>;
     static_assert(!kHbm || !kTab, "HBM-resident state runs the LUT variants");
-    static_assert(!kRich || (kTab && NT <= 256), "register-rich builds: score table, 2- and 4-wave tiles");
+    static_assert(!kRich || ((kTab || kMode == kScore24) && NT <= 256 && !kHbm),
+                  "register-rich builds: 24-bit scores with state in LDS, 2- and 4-wave tiles");
     static_assert(kBits < 64 || kWide, "64-bit words score in int64");
     extern __shared__ __align__(16) unsigned char lds[];
     uint32_t* const ldsWords = reinterpret_cast<uint32_t*>(lds + plan.wordsOff);
@@ -776,19 +778,30 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                     }
                     if constexpr (kRich) {
                         // The register-rich build issues the block's eight
-                        // table reads together, ahead of every use (with few
-                        // waves per SIMD nothing else hides their latency).
-                        int32_t gv[8];
+                        // table (or LUT) reads together, ahead of every use
+                        // (with few waves per SIMD nothing else hides their
+                        // latency).
+                        uint32_t gv[8];
 #pragma unroll
                         for (int j = 0; j < 8; ++j) {
-                            const uint32_t off = ((j & 1) ? (o4[j >> 1] >> 16) : o4[j >> 1]) & 0x1FFCu;
-                            gv[j] = *reinterpret_cast<const int32_t*>(reinterpret_cast<const unsigned char*>(gtab) + off);
+                            const uint32_t off = ((j & 1) ? (o4[j >> 1] >> 16) : o4[j >> 1]) & (kTab ? 0x1FFCu : 0x3Cu);
+                            const unsigned char* base = kTab ? reinterpret_cast<const unsigned char*>(gtab)
+                                                             : reinterpret_cast<const unsigned char*>(lutb);
+                            gv[j] = *reinterpret_cast<const uint32_t*>(base + off);
                         }
                         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                         for (int j = 0; j < 8; ++j) {
                             if (tail && 8 * b + j >= n) continue;
-                            const uint32_t t = static_cast<uint32_t>(__mul24(beta, static_cast<int32_t>(blk.field(j))) + gv[j]);
+                            uint32_t t;
+                            if constexpr (kTab) {
+                                t = static_cast<uint32_t>(__mul24(beta, static_cast<int32_t>(blk.field(j))) +
+                                                          static_cast<int32_t>(gv[j]));
+                            } else {
+                                const uint32_t r = ((j & 1) ? (o[j >> 1] >> 16) : o[j >> 1]);
+                                t = static_cast<uint32_t>(
+                                    scoreOf<kMul24>(beta, static_cast<uint32_t>(blk.field(j)), gv[j], (r >> 4) & 0x7Fu));
+                            }
                             acc[j] = max(acc[j], static_cast<int32_t>((t << cb) + c));
                         }
                         return;
@@ -1232,7 +1245,7 @@ hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s) {
 // state in LDS or HBM.
 template <int kBits, int kMode, bool kHbm>
 static const void* kernelFor(int threads, bool rich) {
-    if constexpr (kMode == dev::kScoreTab && !kHbm) {
+    if constexpr ((kMode == dev::kScoreTab || kMode == dev::kScore24) && !kHbm) {
         if (rich && threads == 128) return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 128, kHbm, true>);
         if (rich && threads == 256) return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 256, kHbm, true>);
     }
@@ -1252,7 +1265,7 @@ static const void* kernelForMode(int mode, int threads, bool rich) {
             if constexpr (kBits <= 32) return kernelFor<kBits, dev::kScore32, kHbm>(threads, false);
             return nullptr;
         case dev::kScore24:   // 24-bit fields only (8/16-bit words)
-            if constexpr (kBits <= 16) return kernelFor<kBits, dev::kScore24, kHbm>(threads, false);
+            if constexpr (kBits <= 16) return kernelFor<kBits, dev::kScore24, kHbm>(threads, rich);
             return nullptr;
         case dev::kScore64: return kernelFor<kBits, dev::kScore64, kHbm>(threads, false);
         case dev::kScoreTab:   // 8-bit words with LDS-resident state only

@@ -96,9 +96,10 @@ namespace launch {
 // Persistent slice kernel over `nItems` chain items, each advanced by at most
 // `sliceSweeps` sweeps, `threads` threads per workgroup; `sched` is a 2-word
 // device counter pair, zero before the launch and left zero after it.
-// `hbm`: the chains' state stays in HBM (no LDS copy).  `rich`: the score
-// table's 2- and 4-wave tiles built for 5 resident workgroups per CU instead
-// of 6 (more registers for the scoring loop's schedule); ignored elsewhere.
+// `hbm`: the chains' state stays in HBM (no LDS copy).  `rich`: the 24-bit
+// score variants' 2- and 4-wave tiles (state in LDS) built for 5 resident
+// workgroups per CU instead of 6 (more registers for the scoring loop's
+// schedule); ignored elsewhere.
 hipError_t anneal(int bits, int mode, int threads, bool hbm, bool rich, const dev::ChainItem* items, int nItems,
                   uint32_t* sched, const dev::LdsPlan& plan, int grid, uint64_t sliceSweeps,
                   hipStream_t s);

@@ -510,12 +510,12 @@ void Context::finishGroup(LaunchGroup& g, int64_t maxN, int maxP, const LaunchKn
             }
         }
     }
-    // The score table's 2- and 4-wave tiles have a register-rich build (5
-    // workgroups per CU instead of 6): taken when it keeps every chain
-    // resident that the lean one would.
+    // The 24-bit score variants' 2- and 4-wave tiles have a register-rich
+    // build (5 workgroups per CU instead of 6): taken when it keeps every
+    // chain resident that the lean one would.
     g.rich = false;
-    if (g.mode == dev::kScoreTab && g.threads <= 256) {
-        const int occRich = occupancy(8, dev::kScoreTab, g.threads, g.plan.total, false, true);
+    if ((g.mode == dev::kScoreTab || g.mode == dev::kScore24) && !g.hbm && g.threads <= 256) {
+        const int occRich = occupancy(g.bits, g.mode, g.threads, g.plan.total, false, true);
         if (occRich > 0 && residentOf(occRich) >= residentOf(occ)) {
             g.rich = true;
             occ = occRich;

@@ -286,7 +286,7 @@ private:
         int maxP = 1;
         uint64_t key = 0;    // groupKey: variant and size class
         bool hbm = false;
-        bool rich = false;   // score table built for 5 workgroups per CU (launch::anneal)
+        bool rich = false;   // 24-bit-score build for 5 workgroups per CU (launch::anneal)
     };
     // Identity of a launch group across plans: word bits, score mode before
     // the table choice, state in HBM, and the size class of its largest n.

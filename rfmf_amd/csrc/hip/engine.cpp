@@ -837,7 +837,6 @@ float Context::runToCompletion(const std::vector<ResidentJob*>& jobs, const Laun
 float Context::runUntil(const std::vector<ResidentJob*>& jobs, const LaunchKnobs& knobsIn, uint64_t until) {
     float total = 0.0f;
     LaunchKnobs knobs = knobsIn;
-    last_ = LaunchReport{};
     // The smallest sweep of a chain that still has sweeps to run before `until`.
     auto lowest = [&](uint64_t& at) {
         bool any = false;
@@ -859,6 +858,8 @@ float Context::runUntil(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
                                                              : room);
     };
     uint64_t at = 0;
+    // A call with nothing left to run keeps the previous launch's report.
+    if (lowest(at)) last_ = LaunchReport{};
     if (knobs.autotune && knobs.threads == 0 && !knobs.hipGraph && lockstep(jobs)) {
         // Every chain is at the same sweep of the same phase list: tune the
         // geometry per phase (the best tile changes with the group count)

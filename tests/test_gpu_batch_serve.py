@@ -253,7 +253,10 @@ def test_windows_with_per_phase_autotune_equal_one_shot(graphs):
     with R.Context(0) as ctx:
         gid = ctx.upload_graph(csr)
         ms = ctx.batch_run_windows([(gid, job)], [50, 170], autotune=True)
+        geo = ctx.last_launch()
     assert len(ms) == 2 and all(m > 0 for m in ms)
+    # the report describes the last launch made, not the empty final call
+    assert geo["threads_per_chain"] in (128, 256, 512, 1024) and geo["wg_per_cu"] >= 1, geo
     same(job, one_shot(job), "windows")
 
 

@@ -293,3 +293,23 @@ def test_header_is_c_and_struct_layouts_match_ctypes(tmp_path):
         assert got[(cname, "size")] == C.sizeof(mirror), cname
         for field, _ in mirror._fields_:
             assert got[(cname, field)] == getattr(mirror, field).offset, f"{cname}.{field}"
+
+
+def test_group_map_reciprocal_needs_one_correction():
+    """The kernel's x mod P for P not a power of two: q = mulhi(x, m) with
+    m = floor((2^32 - 1) / P) is floor(x / P) or one less for every x < 2^31,
+    so r = x - q * P needs at most one subtraction of P (anneal_kernels.hip,
+    modBy).  Checked for every P up to 4096 on random and edge values of x."""
+    rng = np.random.default_rng(7)
+    xs = np.concatenate([rng.integers(0, 2**31, 20000, dtype=np.int64),
+                         np.array([0, 1, 2**31 - 1, 2**31 - 2, 2**24, 2**16 - 1], dtype=np.int64)])
+    xs = xs.astype(object)
+    for p in range(3, 4097):
+        if p & (p - 1) == 0:
+            continue
+        m = (2**32 - 1) // p
+        for x in xs[:: max(1, p // 64)] if p > 64 else xs:
+            q = (x * m) >> 32
+            r = x - q * p
+            assert 0 <= r < 2 * p, (x, p)
+            assert (r - p if r >= p else r) == x % p, (x, p)

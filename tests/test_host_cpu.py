@@ -313,3 +313,37 @@ def test_group_map_reciprocal_needs_one_correction():
             r = x - q * p
             assert 0 <= r < 2 * p, (x, p)
             assert (r - p if r >= p else r) == x % p, (x, p)
+
+
+def test_packed_lane_key_order_and_decode():
+    """The scoring loop's per-lane key (anneal_kernels.hip, scorePackedLanes):
+    with the table variant's score t = beta * (delta + 64) + gumbel, the key
+    ((t << k) + (K - 1 - i) - ((64 * beta) << k)) mod 2^32, read as int32,
+    equals s * 2^k + (K - 1 - i) for the true score s whenever
+    |beta| * B + max|gumbel| < 2^(31 - k); its maximum over a lane's blocks
+    is the largest score, ties to the earliest block; and decoding recovers
+    s and i."""
+    rng = np.random.default_rng(11)
+    gmin, gmax = -1303301, 3617613
+    for k in range(0, 8):
+        K = 1 << k
+        for _ in range(300):
+            B = int(rng.integers(1, 64))
+            lim = (2 ** (31 - k) - gmax - 1) // B
+            beta = int(rng.integers(0, min(lim, 2**23 - 1) + 1))
+            items = []
+            for i in range(K):
+                delta = int(rng.integers(-B, B + 1))
+                g = int(rng.integers(gmin, gmax + 1))
+                s = beta * delta + g
+                assert abs(s) < 2 ** (31 - k)
+                t = beta * (delta + 64) + g
+                key = (((t << k) + (K - 1 - i) - ((64 * beta) << k)) % 2**32)
+                key = key - 2**32 if key >= 2**31 else key
+                assert key == s * 2**k + (K - 1 - i)
+                assert -(2**31) < key < 2**31
+                assert key >> k == s and K - 1 - (key & (K - 1)) == i
+                items.append((s, i, key))
+            best = max(items, key=lambda e: e[2])
+            want = max(items, key=lambda e: (e[0], -e[1]))
+            assert best[:2] == want[:2]

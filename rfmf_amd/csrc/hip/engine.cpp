@@ -671,8 +671,10 @@ Context::SliceHandle Context::launchSlice(const std::vector<ResidentJob*>& jobs,
         }
         hipCheck(hipEventCreate(&h.ev0), "hipEventCreate");
         hipCheck(hipEventCreate(&h.ev1), "hipEventCreate");
-        hipCheck(hipEventRecord(h.ev0, stream_), "hipEventRecord");
         hipCheck(hipMemcpyAsync(slot.dev.get(), staged, bytes, hipMemcpyHostToDevice, stream_), "upload items");
+        // The bracket holds the kernel launches only (the item upload is
+        // ordered before it on the same stream).
+        hipCheck(hipEventRecord(h.ev0, stream_), "hipEventRecord");
         std::vector<dev::ChainItem*> ptrs;
         at = 0;
         for (const LaunchGroup& g : groups) {

@@ -146,3 +146,22 @@ def csr_digest(csr: AnnealCsr) -> str:
     for a in (csr.offsets, csr.edge_to, csr.edge_weight):
         h.update(np.ascontiguousarray(a).tobytes())
     return h.hexdigest()[:16]
+
+
+def request_stream(count: int, qps: float, sweeps_lo: int, sweeps_hi: int, seed: int = 2026,
+                   graph_seed: int = 7, chains_lo: int = 32, chains_hi: int = 256):
+    """A serving stream: `count` solve requests with Poisson arrivals at `qps`
+    (exponential gaps), each a graph of the mixed-batch generator (n 800 to
+    20000) with a uniform chain count in [chains_lo, chains_hi] and a
+    log-uniform sweep count in [sweeps_lo, sweeps_hi] rounded down to a
+    multiple of 8 (equal sweeps for each P of the ladder 128..1).  Returns
+    ([(family, csr, chains, sweeps)], arrival seconds from the first)."""
+    rng = np.random.default_rng(seed)
+    graphs = batch_instances(count, seed=graph_seed)
+    reqs = []
+    for fam, g in graphs:
+        chains = int(rng.integers(chains_lo, chains_hi + 1))
+        sweeps = max(8, int(np.exp(rng.uniform(np.log(sweeps_lo), np.log(sweeps_hi)))) // 8 * 8)
+        reqs.append((fam, g, chains, sweeps))
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, count))
+    return reqs, arrivals - arrivals[0]

@@ -25,12 +25,22 @@ def pytest_collection_modifyitems(config, items):
             it.add_marker(skip)
 
 
+def oracle_threads() -> int:
+    """Host threads for the oracle: the CPUs this process may use, at most
+    OMP_NUM_THREADS when set and at most 16."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, min(16, n))
+
+
 def run_oracle_chains(csr, init, solver_seeds, beta, phases, iters):
     """CPU oracle over every chain (multi-threaded); returns (end, best, cur, best_vals)."""
     from oracle import oracle as O
     end, best, cv, bv, _ = O.anneal_many(csr.offsets, csr.edge_to, csr.edge_weight, init, solver_seeds,
                                          beta.run_end, beta.run_val, beta.chain_run_off, phases, iters,
-                                         threads=min(8, os.cpu_count() or 1))
+                                         threads=oracle_threads())
     return end, best, cv, bv
 
 

@@ -1,5 +1,6 @@
-"""Writes tests/golden/committed_chains.json: per-chain end states taken from
-the reference's committed benchmark logs (results/<run>/logs/<G>.out, .err).
+"""Writes tests/golden/committed_chains.json and committed_digests.json:
+per-chain end states taken from the reference's committed benchmark logs
+(results/<run>/logs/<G>.out, .err).
 
 Each record holds the chain's resolved seed, t0, phases, iters, the best and
 final cut values printed by the reference binary (host recompute), the
@@ -12,6 +13,7 @@ Run in the build container (reads /root/reference):
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import re
 from pathlib import Path
@@ -20,6 +22,8 @@ import numpy as np
 
 REF = Path("/root/reference/results")
 OUT = Path(__file__).resolve().parent / "committed_chains.json"
+DIGESTS = Path(__file__).resolve().parent / "committed_digests.json"
+DIGEST_RUNS = ["V100-SXM2-10M", "V100-PCIE-10M"]
 
 # (run, instance, chain indices): every chain of G62 at 80M sweeps, and a few
 # chains of the other families.
@@ -39,6 +43,10 @@ KER = re.compile(r"^chain (\d+): kernel-best-val: (-?\d+) kernel-final-val: (-?\
 def pack(bits: str) -> str:
     a = np.frombuffer(bits.encode(), dtype=np.uint8) - ord("0")
     return np.packbits(a).tobytes().hex()
+
+
+def short_digest(packed_hex: str) -> str:
+    return hashlib.sha256(bytes.fromhex(packed_hex)).hexdigest()[:16]
 
 
 def parse(run: str, inst: str):
@@ -81,6 +89,24 @@ def main():
     OUT.write_text(json.dumps({"source": "reference results/<run>/logs/<instance>.{out,err}",
                                "chains": recs}, indent=0))
     print(f"wrote {len(recs)} chains to {OUT} ({OUT.stat().st_size} bytes)")
+    runs = {}
+    for run in DIGEST_RUNS:
+        insts = sorted((p.stem for p in (REF / run / "logs").glob("*.out")), key=lambda s: int(s[1:]))
+        runs[run] = {}
+        for inst in insts:
+            chains = parse(run, inst)
+            runs[run][inst] = {
+                "phases": chains[0]["phases"], "iters": chains[0]["iters"], "n": chains[0]["n"],
+                "chains": [[c["seed"], c["t0"], c["best_cut"], c["final_cut"], c["kernel_best"], c["kernel_final"],
+                            short_digest(c["best_partition"]), short_digest(c["final_partition"])]
+                           for c in chains]}
+    DIGESTS.write_text(json.dumps({"source": "reference results/<run>/logs/<instance>.{out,err}",
+                                   "fields": ["seed", "t0", "best_cut", "final_cut", "kernel_best",
+                                              "kernel_final", "best_partition_sha256_16",
+                                              "final_partition_sha256_16"],
+                                   "runs": runs}, separators=(",", ":")))
+    print(f"wrote {sum(len(v['chains']) for r in runs.values() for v in r.values())} chain digests to {DIGESTS} "
+          f"({DIGESTS.stat().st_size} bytes)")
 
 
 if __name__ == "__main__":

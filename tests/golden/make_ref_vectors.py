@@ -90,7 +90,7 @@ def main():
     out["beta_fixed"] = bf
 
     csrs = []
-    for g in ["G1", "G11", "G61", "G62", "G70", "G81"]:
+    for g in sorted((p.name for p in (HERE / "gset").iterdir()), key=lambda s: int(s[1:])):
         lines = run("csr", HERE / "gset" / g).splitlines()
         hdr = lines[0].split()
         off = np.array(lines[1].split(), np.uint32)

@@ -1,12 +1,17 @@
 """Replays chains of the reference's committed 80M-sweep benchmark runs on the
 GPU and requires the committed best and final partitions byte-for-byte.
 
-The fixture (tests/golden/committed_chains.json) holds, per chain, the seed,
-t0, P ladder, sweeps per phase, the cut values and both partition strings the
-reference binary printed (results/V100-*/logs/*.out).  A full replay is 8e7
-sweeps per chain, minutes of GPU time, so the test is opt-in:
-RFMF_LONG=1 selects all 80 G62 chains; RFMF_LONG_INSTANCES=G62,G70,G61,G81
-adds the other families."""
+tests/golden/committed_chains.json holds, per chain, the seed, t0, P ladder,
+sweeps per phase, the cut values and both partition strings the reference
+binary printed (results/V100-*/logs/*.out); committed_digests.json holds
+every chain of the two 8e7-sweep runs (V100-SXM2-10M, V100-PCIE-10M: 12
+instances x 80 chains each) with SHA-256 prefixes of the packed partitions.
+A full replay is 8e7 sweeps per chain, minutes of GPU time, so the tests are
+opt-in: RFMF_LONG=1 with RFMF_LONG_INSTANCES (default G62; "" for none)
+replays one instance's chains of the fixture with full partitions, and with
+RFMF_LONG_RUNS=V100-SXM2-10M (or V100-PCIE-10M) every chain of that run, all
+12 instances in one batch."""
+import hashlib
 import json
 import os
 import time
@@ -27,7 +32,11 @@ def unpack(hexstr: str, n: int) -> np.ndarray:
     return np.where(bits > 0, 1, -1).astype(np.int8)
 
 
-@pytest.mark.parametrize("instance", os.environ.get("RFMF_LONG_INSTANCES", "G62").split(","))
+def _env_list(name, default):
+    return [x for x in os.environ.get(name, default).split(",") if x]
+
+
+@pytest.mark.parametrize("instance", _env_list("RFMF_LONG_INSTANCES", "G62"))
 def test_replay_committed_chains(instance):
     recs = [c for c in json.loads((GOLDEN / "committed_chains.json").read_text())["chains"]
             if c["instance"] == instance and c["run"] == "V100-SXM2-10M"]
@@ -66,4 +75,63 @@ def test_replay_committed_chains(instance):
             bad.append(c["chain"])
     print(f"[replay] {instance}: {len(recs) - len(bad)}/{len(recs)} chains byte-identical to the committed "
           f"partitions; mismatching chains: {bad}", flush=True)
+    assert not bad
+
+
+def partition_digest(spins: np.ndarray) -> str:
+    """First 16 hex digits of SHA-256 over the partition string's bits packed
+    MSB first ('1' for spin +1), as the fixture stores them."""
+    return hashlib.sha256(np.packbits((spins > 0).astype(np.uint8)).tobytes()).hexdigest()[:16]
+
+
+@pytest.mark.parametrize("run_name", _env_list("RFMF_LONG_RUNS", ""))
+def test_replay_every_committed_chain_of_a_run(run_name):
+    """Every chain of one committed 8e7-sweep run (12 Gset instances x 80
+    chains) annealed together in one batch, each required equal to the
+    reference's printed partitions and cut values."""
+    fx = json.loads((GOLDEN / "committed_digests.json").read_text())
+    fields = fx["fields"]
+    insts = fx["runs"][run_name]
+    jobs, recs = [], []
+    for inst, rec in insts.items():
+        csr, _ = R.read_instance_from_file(str(GSET / inst))
+        assert csr.n == rec["n"], inst
+        chains = [dict(zip(fields, c)) for c in rec["chains"]]
+        seeds = [c["seed"] for c in chains]
+        init = np.stack([R.generate_random_spins(s, csr.n) for s in seeds])
+        solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+        beta = R.build_beta_run_tables([c["t0"] for c in chains], sum(rec["iters"]))
+        jobs.append((csr, R.Job(csr, init, solver, beta, rec["phases"], rec["iters"])))
+        recs.append((inst, chains))
+    total = max(int(sum(j.iters)) for _, j in jobs)
+    nchains = sum(j.chains for _, j in jobs)
+    slice_sweeps = 1_000_000
+    print(f"[replay-run] {run_name}: {len(jobs)} instances, {nchains} chains x {total} sweeps in one batch",
+          flush=True)
+    t0 = time.time()
+    kernel_ms = 0.0
+    with R.Context(0) as ctx:
+        gids = [ctx.upload_graph(csr) for csr, _ in jobs]
+        with R.Run(ctx, [(gid, j) for gid, (_, j) in zip(gids, jobs)]) as run:
+            done, at = False, 0
+            while not done:
+                ms, done = run.advance(slice_sweeps)
+                kernel_ms += ms
+                at = min(total, at + slice_sweeps)
+                print(f"[replay-run] {run_name}: {at}/{total} sweeps, {time.time() - t0:.0f} s", flush=True)
+            run.collect()
+    print(f"[replay-run] {run_name}: kernel {kernel_ms / 1e3:.1f} s, "
+          f"{nchains * total / (kernel_ms / 1e3):.3e} chain-steps/s", flush=True)
+    bad = []
+    for (inst, chains), (_, job) in zip(recs, jobs):
+        for c, r in enumerate(chains):
+            ok = (partition_digest(job.best_spins[c]) == r["best_partition_sha256_16"] and
+                  partition_digest(job.end_spins[c]) == r["final_partition_sha256_16"] and
+                  job.best_vals[c] == r["best_cut"] and job.final_vals[c] == r["final_cut"])
+            if not ok:
+                bad.append(f"{inst}:{c}")
+        print(f"[replay-run] {inst}: best {int(job.best_vals.max())} (reference {max(r['best_cut'] for r in chains)})",
+              flush=True)
+    print(f"[replay-run] {run_name}: {nchains - len(bad)}/{nchains} chains byte-identical to the committed "
+          f"partitions; mismatching: {bad[:40]}", flush=True)
     assert not bad

@@ -1,0 +1,156 @@
+"""The benchmark configurations' own workloads on the GPU, checked against the
+CPU oracle chain by chain (bit-exact spins, cuts and best cuts):
+
+  * the headline torus 100 x 200 +/-1 at the headline launch shape (1024
+    chains, 256 threads per chain, score table, register-rich build), a
+    sample of chains over 10^4 sweeps;
+  * the 256-graph mixed batch from the seeded generator (64 chains each) in
+    one hipGraph-captured launch with the autotuner on: every chain against
+    the oracle and against its own one-graph-per-launch run;
+  * a Poisson stream of 24 solve requests from the same generator (32-256
+    chains, 10^2-10^3 sweeps) through the serving loop under an HBM budget
+    that forces queueing: every chain of every request against the oracle.
+
+A chain's result depends only on its inputs (graph, initial spins, solver
+seed, beta schedule, P ladder), so "the same alone and in the mix" means:
+equal to the oracle's single-chain run and to the chain's own launch.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import rfmf_amd as R
+from rfmf_amd import workloads as W
+from conftest import run_oracle_chains
+
+pytestmark = pytest.mark.gpu
+
+LADDER = [128, 64, 32, 16, 8, 4, 2, 1]
+
+
+def make_job(csr, seeds, iters):
+    init = np.stack([R.generate_random_spins(s, csr.n) for s in seeds])
+    solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+    beta = R.build_linear_beta_table(0.1, 3.0, sum(iters), len(seeds))
+    return R.Job(csr, init, solver, beta, LADDER, iters)
+
+
+def oracle_check(job, rows, label):
+    """Oracle on chains `rows` of `job` (same inputs), compared bit for bit."""
+    rows = np.asarray(rows)
+    beta = R.build_linear_beta_table(0.1, 3.0, int(sum(job.iters)), rows.size)
+    end, best, cv, bv = run_oracle_chains(job.csr, job.init[rows], job.seeds[rows], beta, job.phases, job.iters)
+    for k, c in enumerate(rows):
+        assert np.array_equal(job.end_spins[c], end[k]), f"{label} chain {c}: final spins differ from the oracle"
+        assert np.array_equal(job.best_spins[c], best[k]), f"{label} chain {c}: best spins differ from the oracle"
+        assert job.final_vals[c] == cv[k] and job.best_vals[c] == bv[k], f"{label} chain {c}: cuts differ"
+
+
+def test_headline_torus_at_headline_launch_shape_matches_oracle():
+    csr = W.torus(100, 200, seed=0)
+    assert W.csr_digest(csr) == "52f133f8b6e30961"
+    sweeps = 10_000
+    iters = [sweeps // 8] * 8
+    job = make_job(csr, range(1, 1025), iters)
+    print(f"[config2] torus 100x200 +/-1, 1024 chains x {sweeps} sweeps, 256 threads, score table", flush=True)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ctx.batch_run([(gid, job)], threads=256, score_table=1)
+        geo = ctx.last_launch()
+    assert geo["threads_per_chain"] == 256 and geo["score_mode"] == "mul24-table" and geo["word_bits"] == 8
+    assert geo["wg_per_cu"] * 256 >= 1024, geo   # every chain resident at once
+    for c in range(1024):
+        assert R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c]
+        assert R.cut_value_csr(csr, job.end_spins[c]) == job.final_vals[c]
+    rows = list(range(0, 1024, 64)) + [1, 511, 1023]
+    print(f"[config2] oracle on {len(rows)} chains", flush=True)
+    oracle_check(job, rows, "torus")
+
+
+@pytest.fixture(scope="module")
+def batch256():
+    t = time.time()
+    graphs = W.batch_instances(256, seed=0)
+    print(f"[config4] generated 256 graphs in {time.time() - t:.1f} s, sum n = {sum(g.n for _, g in graphs)}",
+          flush=True)
+    return graphs
+
+
+def test_batch_of_256_generated_graphs_matches_oracle_and_plain_mode(batch256):
+    sweeps = 384
+    iters = [sweeps // 8] * 8
+    jobs = [make_job(g, range(64 * i + 1, 64 * i + 65), iters) for i, (_, g) in enumerate(batch256)]
+    plain = [make_job(g, range(64 * i + 1, 64 * i + 65), iters) for i, (_, g) in enumerate(batch256)]
+    print(f"[config4] 256 graphs x 64 chains x {sweeps} sweeps: one hipGraph batch with autotune, then "
+          f"one launch per graph", flush=True)
+    with R.Context(0) as ctx:
+        gids = [ctx.upload_graph(g) for _, g in batch256]
+        ms_b = ctx.batch_run(list(zip(gids, jobs)), slice_sweeps=sweeps, use_hip_graph=True, autotune=True)
+        geo = ctx.last_launch()
+        ms_p = sum(ctx.batch_run([(gid, j)]) for gid, j in zip(gids, plain))
+    print(f"[config4] batched {ms_b:.1f} ms ({geo}), plain {ms_p:.1f} ms", flush=True)
+    for i, (a, b) in enumerate(zip(jobs, plain)):
+        assert np.array_equal(a.end_spins, b.end_spins) and np.array_equal(a.best_spins, b.best_spins), i
+        assert np.array_equal(a.best_vals, b.best_vals) and np.array_equal(a.final_vals, b.final_vals), i
+    t = time.time()
+    for i, ((fam, g), j) in enumerate(zip(batch256, jobs)):
+        oracle_check(j, range(64), f"graph {i} ({fam}, n={g.n})")
+    print(f"[config4] all 16384 chains equal the oracle ({time.time() - t:.1f} s of oracle time)", flush=True)
+
+
+def test_poisson_request_stream_served_matches_oracle():
+    # 2000 requests/s: the stream arrives faster than the device serves it,
+    # and the HBM budget holds about one and a half of the largest requests,
+    # so requests queue for memory while others run.
+    stream, arrivals = W.request_stream(24, 2000.0, 100, 1000, seed=11, graph_seed=13)
+    jobs = [make_job(g, range(1000 * i + 1, 1000 * i + 1 + chains), [sweeps // 8] * 8)
+            for i, (_, g, chains, sweeps) in enumerate(stream)]
+    fps = [R.footprint_bytes(j.csr.n, j.csr.m2, j.chains, j.beta.run_end.size, len(j.phases)) for j in jobs]
+    top = sorted(fps)
+    budget = top[-1] + top[-2] // 2
+    assert sum(fps) > 2 * budget
+    print(f"[config5] 24 requests, chains {min(j.chains for j in jobs)}-{max(j.chains for j in jobs)}, sweeps "
+          f"{min(int(sum(j.iters)) for j in jobs)}-{max(int(sum(j.iters)) for j in jobs)}, footprints "
+          f"{sum(fps)} B in all, budget {budget} B", flush=True)
+    with R.Context(0) as ctx:
+        with R.Server(ctx, budget, slice_sweeps=64) as srv:
+            t0 = time.perf_counter()
+            ids = []
+            for i, j in enumerate(jobs):
+                delay = arrivals[i] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(srv.submit(j))
+            for rid in ids:
+                srv.wait(rid, 120.0)
+            times = [srv.poll(rid) for rid in ids]
+            st = srv.stats()
+    assert st["completed"] == 24 and st["rejected"] == 0
+    assert all(d for d, *_ in times)
+    ts = np.array([t[1] for t in times])
+    ta = np.array([t[2] for t in times])
+    td = np.array([t[3] for t in times])
+    assert np.all(ts <= ta) and np.all(ta <= td)
+    # Admission invariant: the requests resident when one is admitted (theirs
+    # included) fit the budget.  (A request counts as resident from its
+    # admission time, taken after its upload, to its completion time.)
+    for i in range(24):
+        live = (ta <= ta[i]) & (ta[i] < td)
+        assert sum(f for f, l in zip(fps, live) if l) <= budget, i
+    # Overlap: two requests resident on the device at once.
+    order = np.argsort(ta)
+    assert any(ta[order[k + 1]] < td[order[k]] for k in range(23))
+    # Queueing: the requests submitted before the first completion do not all
+    # fit at once, so one of them is admitted only after it.
+    first = td.min()
+    early = ts < first
+    print(f"[config5] {int(early.sum())} requests submitted before the first completion "
+          f"({sum(f for f, e in zip(fps, early) if e)} B); waits {np.round(ta - ts, 4).tolist()}", flush=True)
+    if sum(f for f, e in zip(fps, early) if e) > budget:
+        assert np.any(early & (ta > first))
+    t = time.time()
+    for i, j in enumerate(jobs):
+        oracle_check(j, range(j.chains), f"request {i} (n={j.csr.n}, {j.chains} chains)")
+    print(f"[config5] every chain of every request equals the oracle ({time.time() - t:.1f} s)", flush=True)

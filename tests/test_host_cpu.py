@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     assert R.lib().rfmf_version().decode().startswith("rfmf")
 
 
-@pytest.mark.parametrize("inst", ["G1", "G11", "G61", "G62", "G70", "G81"])
+@pytest.mark.parametrize("inst", sorted((p.name for p in GSET.iterdir()), key=lambda s: int(s[1:])))
 def test_csr_product_equals_oracle(inst):
     csr, st = R.read_instance_from_file(str(GSET / inst))
     off, to, w, hn, hm = O.read_gset(GSET / inst)

@@ -182,7 +182,7 @@ def test_gate_boundary_closed_form():
 
 # ---------------------------------------------------------------- graphs and cuts
 
-@pytest.mark.parametrize("inst", ["G1", "G11", "G61", "G62", "G70", "G81"])
+@pytest.mark.parametrize("inst", sorted((p.name for p in GSET.iterdir()), key=lambda s: int(s[1:])))
 def test_csr_matches_reference_code(inst):
     off, to, w, hn, hm = O.read_gset(GSET / inst)
     rec = next(r for r in REF["csr"] if r["instance"] == inst)
@@ -225,6 +225,32 @@ def test_committed_partitions_rescore_to_committed_values():
         c = next(x for x in COMMITTED if x["instance"] == rec["instance"] and x["run"] == rec["run"]
                  and x["chain"] == rec["chain"])
         assert rec["cut"] == c["best_cut" if rec["which"] == "best_partition" else "final_cut"]
+
+
+def test_committed_digest_fixture_agrees_with_partitions_and_results_tables():
+    # The digest fixture (every chain of both 8e7-sweep runs) against the
+    # full-partition fixture and against the runs' results.csv rows: same
+    # seeds and t0s in order, the table's best cut and its hit count.
+    dig = json.loads((GOLDEN / "committed_digests.json").read_text())
+    runs = json.loads((GOLDEN / "reference_runs.json").read_text())
+    fields = dig["fields"]
+    for run, insts in dig["runs"].items():
+        assert len(insts) == 12, run
+        for inst, rec in insts.items():
+            chains = [dict(zip(fields, c)) for c in rec["chains"]]
+            row = runs[run][inst]
+            assert [c["seed"] for c in chains] == row["seeds"] and [c["t0"] for c in chains] == row["t0s"]
+            assert rec["phases"] == row["phases"] and rec["iters"] == row["iters"]
+            best = max(c["best_cut"] for c in chains)
+            assert best == row["best_cut"], (run, inst)
+            assert sum(c["best_cut"] == best for c in chains) == row["hits_at_best"], (run, inst)
+            assert all(c["kernel_best"] == c["best_cut"] and c["kernel_final"] == c["final_cut"] for c in chains)
+    for c in COMMITTED:
+        d = dict(zip(fields, dig["runs"][c["run"]][c["instance"]]["chains"][c["chain"]]))
+        assert d["seed"] == c["seed"] and d["best_cut"] == c["best_cut"]
+        for key in ("best_partition", "final_partition"):
+            packed = np.packbits(_bits(c[key], c["n"])).tobytes()
+            assert hashlib.sha256(packed).hexdigest()[:16] == d[key + "_sha256_16"], (c["instance"], c["chain"])
 
 
 def test_cut_bruteforce_edge_list():

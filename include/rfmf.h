@@ -156,6 +156,27 @@ rfmf_status rfmf_anneal_tables(const rfmf_csr* g, int64_t max_weighted_deg,
                                int8_t* end_spins, int8_t* best_spins, float* kernel_ms,
                                int64_t* kernel_best_vals, int64_t* kernel_final_vals);
 
+/* rfmf_anneal_tables without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, checks and
+ * results as rfmf_anneal_tables, bit for bit; no hipMalloc, hipFree, pinned
+ * or heap allocation on its path.  All device memory comes from one fixed
+ * 64 MiB array per device, held until the process ends; a call whose
+ * CSR, spins, chain state, tables and staging need more returns
+ * RFMF_ERR_UNSUPPORTED (about C * (3 n + n / 8) + 16 m2 bytes for 8-bit
+ * state words) before any device work.  Calls on one device are serialized
+ * by a per-device lock and share the array: each call overwrites what the
+ * previous one left there, so calls that overlap on one device (from
+ * threads that bypass the lock, e.g. other libraries writing the same
+ * memory) would race.  Runs on the null stream and waits for its work. */
+rfmf_status rfmf_anneal_tables_noalloc(const rfmf_csr* g, int64_t max_weighted_deg,
+                                       const int8_t* initial_spins, int64_t chains,
+                                       const rfmf_beta_tables* beta, const uint64_t* phases,
+                                       const uint64_t* iters, int64_t n_phases,
+                                       const uint64_t* solver_seeds, const rfmf_launch_cfg* cfg,
+                                       int8_t* end_spins, int8_t* best_spins, float* kernel_ms,
+                                       int64_t* kernel_best_vals, int64_t* kernel_final_vals);
+
 /* ---------------------------------------------------------------------- */
 /* Resident device context: graphs in HBM, multi-graph batches, serving.   */
 /* No reference counterpart (the reference allocates per call and runs one */

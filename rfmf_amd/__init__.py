@@ -260,6 +260,23 @@ def anneal_gpu_tables(csr: AnnealCsr, max_weighted_deg: int, initial_spins, beta
     return AnnealResult(end, best, ms.value, bv, fv)
 
 
+def anneal_gpu_tables_noalloc(csr: AnnealCsr, max_weighted_deg: int, initial_spins, beta: BetaRunTables, phases,
+                              iters, solver_seeds, cfg: LaunchConfig | None = None) -> AnnealResult:
+    """anneal_gpu_tables through the allocation-free entry point: the same
+    results, with every byte of device memory taken from the library's fixed
+    per-device array (RfmfError status 2 when the call does not fit it)."""
+    init, seeds, ph, it, C_, (end, best, bv, fv) = _prep_common(csr, initial_spins, solver_seeds, phases, iters)
+    bs = beta.struct()
+    ms = C.c_float()
+    cs = (cfg or LaunchConfig()).struct()
+    check(lib().rfmf_anneal_tables_noalloc(C.byref(csr.view()), int(max_weighted_deg), _ptr(init, C.c_int8), C_,
+                                           C.byref(bs), _ptr(ph, C.c_uint64), _ptr(it, C.c_uint64), ph.size,
+                                           _ptr(seeds, C.c_uint64), C.byref(cs), _ptr(end, C.c_int8),
+                                           _ptr(best, C.c_int8), C.byref(ms), _ptr(bv, C.c_int64),
+                                           _ptr(fv, C.c_int64)))
+    return AnnealResult(end, best, ms.value, bv, fv)
+
+
 def footprint_bytes(n: int, m2: int, chains: int, beta_runs: int, n_phases: int) -> int:
     return int(lib().rfmf_footprint_bytes(n, m2, chains, beta_runs, n_phases))
 

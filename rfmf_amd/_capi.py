@@ -93,6 +93,9 @@ SIGNATURES = {
                               C.POINTER(LaunchCfg), i8p, i8p, f32p, i64p, i64p]),
     "rfmf_anneal_tables": (C.c_int, [C.POINTER(Csr), C.c_int64, i8p, C.c_int64, C.POINTER(BetaTables), u64p,
                                      u64p, C.c_int64, u64p, C.POINTER(LaunchCfg), i8p, i8p, f32p, i64p, i64p]),
+    "rfmf_anneal_tables_noalloc": (C.c_int, [C.POINTER(Csr), C.c_int64, i8p, C.c_int64, C.POINTER(BetaTables),
+                                             u64p, u64p, C.c_int64, u64p, C.POINTER(LaunchCfg), i8p, i8p, f32p,
+                                             i64p, i64p]),
     "rfmf_ctx_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
     "rfmf_ctx_destroy": (None, [C.c_void_p]),
     "rfmf_ctx_upload_graph": (C.c_int, [C.c_void_p, C.POINTER(Csr), i32p]),

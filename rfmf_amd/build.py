@@ -29,7 +29,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-paramet
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 HOST_SOURCES = ["host/graph.cpp", "host/read_gset.cpp", "host/max_cut.cpp", "host/host_common.cpp"]
-HIP_SOURCES = ["hip/anneal_kernels.hip", "hip/engine.cpp", "hip/capi.cpp", "hip/server.cpp", "hip/tune_flips.cpp"]
+HIP_SOURCES = ["hip/anneal_kernels.hip", "hip/engine.cpp", "hip/capi.cpp", "hip/server.cpp", "hip/tune_flips.cpp",
+               "hip/noalloc.hip"]
 
 
 def _run(cmd: list[str]) -> None:

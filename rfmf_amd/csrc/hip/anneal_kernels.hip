@@ -46,6 +46,7 @@
 
 #include "anneal_device.hpp"
 #include "anneal_launch.hpp"
+#include "init_words.hpp"
 
 namespace rfmf {
 namespace dev {
@@ -1172,54 +1173,14 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     }
 }
 
-// Packs initial spins into state words: S[v] = sum_w w * spin[to] over v's
-// sorted row (src/anneal_kernels.cu:251-256), word = spin bit |
-// spin*S + bias; best-spin bitmask = initial spins.  One thread per 32
-// vertices.
+// Packs initial spins into state words and best bits (initWordGroup), one
+// thread per 32 vertices, chain blockIdx.y of the item array.
 template <int kBits>
 __global__ __launch_bounds__(256) void initWords(const InitItem* __restrict__ items, int nItems) {
-    using F = WordFmt<kBits>;
     const InitItem& it = items[blockIdx.y];
-    const int n = it.n;
-    const int nGroups = (n + 31) >> 5;
-    const int2* edges = reinterpret_cast<const int2*>(it.edges);
-    for (int grp = blockIdx.x * blockDim.x + threadIdx.x; grp < nGroups;
-         grp += gridDim.x * blockDim.x) {
-        using Word = WordOf<kBits>;
-        using Delta = DeltaOf<kBits>;
-        uint32_t bits = 0;
-        uint32_t packed[kBits];
-#pragma unroll
-        for (int q = 0; q < kBits; ++q) packed[q] = 0u;
-#pragma unroll 4
-        for (int k = 0; k < 32; ++k) {
-            const int v = 32 * grp + k;
-            Word word = 0u;
-            if (v < n) {
-                Delta S = 0;
-                for (uint32_t i = it.offsets[v]; i < it.offsets[v + 1]; ++i) {
-                    const int2 e = edges[i];
-                    S += static_cast<Delta>(e.y) * static_cast<Delta>(it.spins[e.x]);
-                }
-                const bool up = it.spins[v] > 0;
-                const Delta delta = up ? S : -S;
-                word = (up ? F::kSpin : Word{0}) | static_cast<Word>(delta + static_cast<Delta>(F::kBias));
-                bits |= (up ? 1u : 0u) << k;
-            }
-            if constexpr (kBits == 64) {
-                packed[2 * k] = static_cast<uint32_t>(word);
-                packed[2 * k + 1] = static_cast<uint32_t>(word >> 32);
-            } else if constexpr (kBits == 32) {
-                packed[k] = word;
-            } else {
-                constexpr int per = 32 / kBits;
-                packed[k / per] |= word << (kBits * (k % per));
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kBits; ++q) it.words[grp * kBits + q] = packed[q];
-        it.bestbits[grp] = bits;
-    }
+    const int nGroups = (it.n + 31) >> 5;
+    for (int grp = blockIdx.x * blockDim.x + threadIdx.x; grp < nGroups; grp += gridDim.x * blockDim.x)
+        (void)initWordGroup<kBits>(it.offsets, it.edges, it.spins, it.words, it.bestbits, it.n, grp);
 }
 
 // Plain device copy (16 B per lane, grid-stride): the known-size reference

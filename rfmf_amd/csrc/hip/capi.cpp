@@ -45,6 +45,33 @@ int64_t rfmf::footprintBytes(int64_t n, int64_t m2, int64_t chains, int64_t beta
            beta_runs * 12 + n_phases * 12 + chains * 8;
 }
 
+namespace rfmf {
+
+void passThrough(const rfmf_csr* g, const int8_t* init, int64_t C, int8_t* end, int8_t* best, float* ms,
+                 int64_t* bestVals, int64_t* finalVals) {
+    const int64_t n = g->n;
+    if (end) std::memcpy(end, init, static_cast<size_t>(C * n));
+    if (best) std::memcpy(best, init, static_cast<size_t>(C * n));
+    if (ms) *ms = 0.0f;
+    for (int64_t c = 0; c < C; ++c) {
+        const int64_t v = anneallib::solver::cutValueRaw(g->offsets, g->edge_to, g->edge_weight, n, init + c * n);
+        if (bestVals) bestVals[c] = v;
+        if (finalVals) finalVals[c] = v;
+    }
+}
+
+void checkOneShot(const rfmf_csr* g, const int8_t* init, int64_t C, const uint64_t* phases, const uint64_t* iters,
+                  int64_t nPhases, const uint64_t* seeds) {
+    requireCsr(g);
+    if (C <= 0 || !seeds || (g->n > 0 && !init))
+        throw Error(Status::InvalidArg, "annealGpu needs non-empty, equal-length t0s, solver seeds and "
+                                        "initial spins");
+    if (nPhases < 0 || (nPhases > 0 && (!phases || !iters)))
+        throw Error(Status::InvalidArg, "invalid phases / iters");
+}
+
+}  // namespace rfmf
+
 extern "C" {
 
 const char* rfmf_status_string(rfmf_status s) {
@@ -203,21 +230,6 @@ rfmf_status rfmf_beta_tables_linear(int32_t b0, int32_t b1, int64_t total_steps,
 
 namespace {
 
-// n == 0 or no phases: initial state returned without device work
-// (src/anneal_kernels.cu:856-868).
-void passThrough(const rfmf_csr* g, const int8_t* init, int64_t C, int8_t* end, int8_t* best,
-                 float* ms, int64_t* bestVals, int64_t* finalVals) {
-    const int64_t n = g->n;
-    if (end) std::memcpy(end, init, static_cast<size_t>(C * n));
-    if (best) std::memcpy(best, init, static_cast<size_t>(C * n));
-    if (ms) *ms = 0.0f;
-    for (int64_t c = 0; c < C; ++c) {
-        const int64_t v = anneallib::solver::cutValueRaw(g->offsets, g->edge_to, g->edge_weight, n, init + c * n);
-        if (bestVals) bestVals[c] = v;
-        if (finalVals) finalVals[c] = v;
-    }
-}
-
 void annealOneShot(const rfmf_csr* g, int64_t B, const int8_t* init, int64_t C,
                    const int64_t* runEnd, const int32_t* runVal, const int64_t* runOff, int64_t nRuns,
                    int32_t gateBetaMax, const uint64_t* phases, const uint64_t* iters, int64_t nPhases,
@@ -252,16 +264,6 @@ void annealOneShot(const rfmf_csr* g, int64_t B, const int8_t* init, int64_t C,
     ctx.collect(*job, spec, ctx.stream());
 }
 
-void checkCommon(const rfmf_csr* g, const int8_t* init, int64_t C, const uint64_t* phases,
-                 const uint64_t* iters, int64_t nPhases, const uint64_t* seeds) {
-    requireCsr(g);
-    if (C <= 0 || !seeds || (g->n > 0 && !init))
-        throw Error(Status::InvalidArg, "annealGpu needs non-empty, equal-length t0s, solver seeds and "
-                                        "initial spins");
-    if (nPhases < 0 || (nPhases > 0 && (!phases || !iters)))
-        throw Error(Status::InvalidArg, "invalid phases / iters");
-}
-
 }  // namespace
 
 rfmf_status rfmf_anneal(const rfmf_csr* g, int64_t max_weighted_deg, const int8_t* initial_spins,
@@ -270,10 +272,10 @@ rfmf_status rfmf_anneal(const rfmf_csr* g, int64_t max_weighted_deg, const int8_
                         int8_t* end_spins, int8_t* best_spins, float* kernel_ms, int64_t* kernel_best_vals,
                         int64_t* kernel_final_vals) {
     return guarded([&] {
-        checkCommon(g, initial_spins, chains, phases, iters, n_phases, solver_seeds);
+        rfmf::checkOneShot(g, initial_spins, chains, phases, iters, n_phases, solver_seeds);
         if (!t0s) throw Error(Status::InvalidArg, "null t0s");
         if (g->n == 0 || n_phases == 0) {
-            passThrough(g, initial_spins, chains, end_spins, best_spins, kernel_ms, kernel_best_vals,
+            rfmf::passThrough(g, initial_spins, chains, end_spins, best_spins, kernel_ms, kernel_best_vals,
                         kernel_final_vals);
             return;
         }
@@ -298,15 +300,15 @@ rfmf_status rfmf_anneal_tables(const rfmf_csr* g, int64_t max_weighted_deg, cons
                                const rfmf_launch_cfg* cfg, int8_t* end_spins, int8_t* best_spins,
                                float* kernel_ms, int64_t* kernel_best_vals, int64_t* kernel_final_vals) {
     return guarded([&] {
-        checkCommon(g, initial_spins, chains, phases, iters, n_phases, solver_seeds);
+        rfmf::checkOneShot(g, initial_spins, chains, phases, iters, n_phases, solver_seeds);
         if (!beta || !beta->run_end || !beta->run_val || !beta->chain_run_off || beta->n_runs < 1)
             throw Error(Status::InvalidArg, "invalid beta tables");
         if (g->n == 0 || n_phases == 0) {
-            passThrough(g, initial_spins, chains, end_spins, best_spins, kernel_ms, kernel_best_vals,
+            rfmf::passThrough(g, initial_spins, chains, end_spins, best_spins, kernel_ms, kernel_best_vals,
                         kernel_final_vals);
             return;
         }
-        annealTotalSteps(std::vector<uint64_t>(iters, iters + n_phases));
+        annealTotalSteps(iters, n_phases);
         annealOneShot(g, max_weighted_deg, initial_spins, chains, beta->run_end, beta->run_val,
                       beta->chain_run_off, beta->n_runs, -1, phases, iters, n_phases, solver_seeds, cfg,
                       end_spins, best_spins, kernel_ms, kernel_best_vals, kernel_final_vals);

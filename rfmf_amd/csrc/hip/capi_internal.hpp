@@ -18,6 +18,16 @@ namespace rfmf {
 // Sets the calling thread's rfmf_last_error() message.
 void setLastError(const std::string& msg);
 
+// One-shot argument checks shared by the annealGpu replacements: a valid CSR
+// view, at least one chain with seeds and initial spins, phases and iters
+// present when there are phases.  Throws Error(InvalidArg).
+void checkOneShot(const rfmf_csr* g, const int8_t* init, int64_t C, const uint64_t* phases, const uint64_t* iters,
+                  int64_t nPhases, const uint64_t* seeds);
+// n == 0 or no phases: the initial state returned without device work
+// (src/anneal_kernels.cu:856-868).  Host memory only.
+void passThrough(const rfmf_csr* g, const int8_t* init, int64_t C, int8_t* end, int8_t* best, float* ms,
+                 int64_t* bestVals, int64_t* finalVals);
+
 // HBM bytes a job keeps resident with state words of `bits` bits.
 int64_t footprintBytes(int64_t n, int64_t m2, int64_t chains, int64_t beta_runs, int64_t n_phases, int bits);
 

@@ -228,29 +228,34 @@ uint64_t Context::tuneKey(int bits, int mode, int64_t maxN, int64_t chains, int 
            static_cast<uint64_t>(maxP);
 }
 
-bool Context::fitsLds(int64_t maxN, int bits, int mode, int maxP, int threads, bool hbm) const {
+bool fitsLds(const hipDeviceProp_t& props, int64_t maxN, int bits, int mode, int maxP, int threads, bool hbm) {
     const dev::LdsPlan p = dev::makeLdsPlan(maxN, bits, maxP, threads, mode, hbm);
-    return static_cast<size_t>(p.total) + dev::staticLds(mode) <= props_.sharedMemPerBlock;
+    return static_cast<size_t>(p.total) + dev::staticLds(mode) <= props.sharedMemPerBlock;
 }
 
 bool Context::stateInHbm(int64_t n, int bits, int mode, int maxP, const LaunchKnobs& knobs) const {
     return knobs.stateInHbm > 0 || !fitsLds(n, bits, mode, maxP, 128);
 }
 
+int heuristicThreads(const hipDeviceProp_t& props, int bits, int mode, int64_t maxN, int64_t chains, int maxP,
+                     bool hbm) {
+    // With several chains per CU the 4-wave tile keeps every SIMD busy across
+    // chains (the 2-wave tile for small graphs, and for state in HBM, where
+    // LDS no longer limits the chains per CU); with about one chain per CU or
+    // fewer, 8 waves per chain cut the per-sweep latency (16 waves add more
+    // per-wave work than they hide), and 16 when the state is in HBM, whose
+    // loads want more waves in flight.
+    const double perCu = static_cast<double>(chains) / std::max(1, props.multiProcessorCount);
+    int t = hbm ? 1024 : 512;
+    if (perCu >= 2.0) t = (hbm || maxN <= 2048) ? 128 : 256;
+    while (t > 128 && !fitsLds(props, maxN, bits, mode, maxP, t, hbm)) t /= 2;
+    return t;
+}
+
 int Context::chooseThreads(int bits, int mode, int64_t maxN, int64_t chains, int maxP, bool hbm) const {
     const auto it = tuned_.find(tuneKey(bits, mode, maxN, chains, maxP, hbm));
     if (it != tuned_.end()) return it->second;
-    // Heuristic: with several chains per CU the 4-wave tile keeps every SIMD
-    // busy across chains (the 2-wave tile for small graphs, and for state in
-    // HBM, where LDS no longer limits the chains per CU); with about one chain
-    // per CU or fewer, 8 waves per chain cut the per-sweep latency (16 waves
-    // add more per-wave work than they hide), and 16 when the state is in
-    // HBM, whose loads want more waves in flight.
-    const double perCu = static_cast<double>(chains) / std::max(1, props_.multiProcessorCount);
-    int t = hbm ? 1024 : 512;
-    if (perCu >= 2.0) t = (hbm || maxN <= 2048) ? 128 : 256;
-    while (t > 128 && !fitsLds(maxN, bits, mode, maxP, t, hbm)) t /= 2;
-    return t;
+    return heuristicThreads(props_, bits, mode, maxN, chains, maxP, hbm);
 }
 
 std::unique_ptr<ResidentJob> Context::admit(const JobSpec& spec, hipStream_t stream, bool streamOrdered) {
@@ -474,25 +479,33 @@ uint64_t Context::groupKey(int bits, int mode, bool hbm, int64_t topN, bool spli
 }
 
 void Context::finishGroup(LaunchGroup& g, int64_t maxN, int maxP, const LaunchKnobs& knobs) {
-    // Tile, LDS plan, score arithmetic and grid for one launch group.
     int tuned = 0;
     for (const auto& kv : knobs.groupThreads)
         if (kv.first == g.key) tuned = kv.second;
     g.maxP = maxP;
+    const int64_t nItems = static_cast<int64_t>(g.items.size());
     g.threads = knobs.threads > 0 ? knobs.threads
                 : tuned > 0      ? tuned
-                                 : chooseThreads(g.bits, g.mode, maxN, static_cast<int64_t>(g.items.size()), maxP,
-                                                 g.hbm);
+                                 : chooseThreads(g.bits, g.mode, maxN, nItems, maxP, g.hbm);
+    shapeGroup(this, props_, g, maxN, nItems, knobs);
+}
+
+void shapeGroup(Context* ctx, const hipDeviceProp_t& props, GroupShape& g, int64_t maxN, int64_t nItems,
+                const LaunchKnobs& knobs) {
+    // LDS plan, score arithmetic and grid for one launch group.
+    auto occupancy = [&](int bits, int mode, int threads, size_t lds, bool hbm, bool rich) {
+        return ctx ? ctx->occupancy(bits, mode, threads, lds, hbm, rich)
+                   : launch::maxActiveBlocks(bits, mode, threads, hbm, rich, lds);
+    };
     if (!dev::validThreads(g.threads))
         throw Error(Status::InvalidArg, "threads per chain must be 128, 256, 512 or 1024");
-    g.plan = dev::makeLdsPlan(maxN, g.bits, maxP, g.threads, g.mode, g.hbm);
-    if (static_cast<size_t>(g.plan.total) + dev::staticLds(g.mode) > props_.sharedMemPerBlock)
+    g.plan = dev::makeLdsPlan(maxN, g.bits, g.maxP, g.threads, g.mode, g.hbm);
+    if (static_cast<size_t>(g.plan.total) + dev::staticLds(g.mode) > props.sharedMemPerBlock)
         throw Error(Status::Unsupported, "combined launch needs " + std::to_string(g.plan.total) +
                                              " B of LDS per workgroup");
-    int occ = std::max(1, occupancy(g.bits, g.mode, g.threads, g.plan.total, g.hbm));
-    const int64_t nItems = static_cast<int64_t>(g.items.size());
+    int occ = std::max(1, occupancy(g.bits, g.mode, g.threads, g.plan.total, g.hbm, false));
     auto residentOf = [&](int o) {
-        int64_t r = static_cast<int64_t>(o) * props_.multiProcessorCount;
+        int64_t r = static_cast<int64_t>(o) * props.multiProcessorCount;
         if (knobs.gridCap > 0) r = std::min<int64_t>(r, knobs.gridCap);
         return std::min(r, nItems);
     };
@@ -500,9 +513,9 @@ void Context::finishGroup(LaunchGroup& g, int64_t maxN, int maxP, const LaunchKn
     // 8 KB more static LDS) replaces the per-sweep LUT when it keeps every
     // chain of the group resident that the LUT variant would.
     if (g.bits == 8 && g.mode == dev::kScore24 && !g.hbm && knobs.scoreTable >= 0) {
-        const dev::LdsPlan tp = dev::makeLdsPlan(maxN, 8, maxP, g.threads, dev::kScoreTab);
-        if (static_cast<size_t>(tp.total) + dev::staticLds(dev::kScoreTab) <= props_.sharedMemPerBlock) {
-            const int occTab = occupancy(8, dev::kScoreTab, g.threads, tp.total);
+        const dev::LdsPlan tp = dev::makeLdsPlan(maxN, 8, g.maxP, g.threads, dev::kScoreTab);
+        if (static_cast<size_t>(tp.total) + dev::staticLds(dev::kScoreTab) <= props.sharedMemPerBlock) {
+            const int occTab = occupancy(8, dev::kScoreTab, g.threads, tp.total, false, false);
             if (occTab > 0 && (knobs.scoreTable > 0 || residentOf(occTab) >= residentOf(occ))) {
                 g.mode = dev::kScoreTab;
                 g.plan = tp;

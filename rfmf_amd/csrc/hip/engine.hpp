@@ -167,6 +167,37 @@ struct LaunchKnobs {
     std::vector<std::pair<uint64_t, int>> groupThreads;
 };
 
+// Launch shape of one group of chains that share a kernel variant: tile,
+// LDS plan, score arithmetic, build and grid.
+struct GroupShape {
+    int bits = 8;
+    int mode = dev::kScore32;
+    int threads = dev::kThreads;
+    dev::LdsPlan plan{};
+    int grid = 1;
+    int occupancy = 1;   // resident workgroups per CU
+    int maxP = 1;
+    bool hbm = false;
+    bool rich = false;   // 24-bit-score build for 5 workgroups per CU (launch::anneal)
+};
+
+class Context;
+
+// Whether a workgroup of `threads` threads holds the LDS plan of a chain of
+// maxN vertices (state in LDS unless `hbm`) plus the variant's static LDS.
+bool fitsLds(const hipDeviceProp_t& props, int64_t maxN, int bits, int mode, int maxP, int threads, bool hbm);
+// Threads per chain from chains per CU and n (no tuning): with several
+// chains per CU the 4-wave tile (2-wave for small graphs and HBM-resident
+// state), else 8 waves (16 with state in HBM), halved until the plan fits.
+int heuristicThreads(const hipDeviceProp_t& props, int bits, int mode, int64_t maxN, int64_t chains, int maxP,
+                     bool hbm);
+// Completes a group whose bits, mode, hbm, maxP and threads are set: LDS
+// plan, the score-table and register-rich choices, occupancy and grid for
+// nItems chains.  Occupancies come from `ctx`'s cache when it is given, else
+// from the runtime.  Throws Error(InvalidArg / Unsupported).
+void shapeGroup(Context* ctx, const hipDeviceProp_t& props, GroupShape& g, int64_t maxN, int64_t nItems,
+                const LaunchKnobs& knobs);
+
 // What the last launch of a context used (for reports).
 struct LaunchReport {
     int threads = 0;
@@ -275,18 +306,9 @@ public:
     std::mutex& mutex() { return mu_; }
 
 private:
-    struct LaunchGroup {
+    struct LaunchGroup : GroupShape {
         std::vector<dev::ChainItem> items;
-        int bits = 8;
-        int mode = dev::kScore32;
-        int threads = dev::kThreads;
-        dev::LdsPlan plan{};
-        int grid = 1;
-        int occupancy = 1;   // resident workgroups per CU
-        int maxP = 1;
         uint64_t key = 0;    // groupKey: variant and size class
-        bool hbm = false;
-        bool rich = false;   // 24-bit-score build for 5 workgroups per CU (launch::anneal)
     };
     // Identity of a launch group across plans: word bits, score mode before
     // the table choice, state in HBM, and the size class of its largest n.
@@ -300,7 +322,9 @@ private:
     hipError_t enqueueGroups(const std::vector<LaunchGroup>& groups, const std::vector<dev::ChainItem*>& items,
                              uint64_t slice);
     static uint64_t tuneKey(int bits, int mode, int64_t maxN, int64_t chains, int maxP, bool hbm);
-    bool fitsLds(int64_t maxN, int bits, int mode, int maxP, int threads, bool hbm = false) const;
+    bool fitsLds(int64_t maxN, int bits, int mode, int maxP, int threads, bool hbm = false) const {
+        return rfmf::fitsLds(props_, maxN, bits, mode, maxP, threads, hbm);
+    }
     std::unordered_map<uint64_t, int> tuned_;
     LaunchReport last_;
 

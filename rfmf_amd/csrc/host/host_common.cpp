@@ -9,8 +9,13 @@ using rfmf::Error;
 using rfmf::Status;
 
 int64_t annealTotalSteps(const std::vector<uint64_t>& iters) {
+    return annealTotalSteps(iters.data(), static_cast<int64_t>(iters.size()));
+}
+
+int64_t annealTotalSteps(const uint64_t* iters, int64_t count) {
     uint64_t total = 0;
-    for (uint64_t it : iters) {
+    for (int64_t i = 0; i < count; ++i) {
+        const uint64_t it = iters[i];
         if (it > static_cast<uint64_t>(kMaxTotalSteps) ||
             total + it > static_cast<uint64_t>(kMaxTotalSteps))
             throw Error(Status::StepCap, "Error: --iters totals more than " +

@@ -80,6 +80,8 @@ struct BetaRunTables {
 // Sum of per-phase sweep counts, checked against kMaxTotalSteps in unsigned
 // arithmetic per entry.  Throws rfmf::Error(StepCap).
 int64_t annealTotalSteps(const std::vector<uint64_t>& iters);
+// The same over `count` entries (no copies).
+int64_t annealTotalSteps(const uint64_t* iters, int64_t count);
 
 // One log table per distinct t0 (exact double equality), over 1-based sweeps
 // [1, totalSteps].  Throws rfmf::Error(BetaCap / InvalidArg).

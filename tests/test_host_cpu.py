@@ -190,6 +190,25 @@ def test_anneal_rejects_bad_inputs_before_device_work():
     assert e.value.status == 4
 
 
+def test_noalloc_entry_point_refuses_calls_beyond_its_array_before_device_work():
+    # 4096 chains of the 100 x 200 torus need 82 MB of initial spins alone,
+    # more than the allocation-free entry point's fixed 64 MiB array: refused
+    # with RFMF_ERR_UNSUPPORTED before any device call (this runs without a GPU).
+    csr = W.torus(100, 200, seed=0)
+    C_ = 4096
+    init = np.ones((C_, csr.n), np.int8)
+    beta = R.build_linear_beta_table(0.1, 3.0, 8, C_)
+    with pytest.raises(R.RfmfError) as ei:
+        R.anneal_gpu_tables_noalloc(csr, R.max_weighted_degree(csr), init, beta, [8, 1], [4, 4],
+                                    list(range(1, C_ + 1)))
+    assert ei.value.status == 2 and "scratch" in str(ei.value)
+    # invalid inputs are refused by the same checks as the allocating path
+    with pytest.raises(R.RfmfError) as ei:
+        R.anneal_gpu_tables_noalloc(csr, 4, init[:2] * 0, R.build_linear_beta_table(0.1, 3.0, 8, 2), [8, 1],
+                                    [4, 4], [1, 2])
+    assert ei.value.status == 1 and "spins" in str(ei.value)
+
+
 def test_anneal_passthrough_without_phases():
     csr = _g11()
     init = np.stack([R.generate_random_spins(s, csr.n) for s in (3, 4)])

@@ -43,8 +43,8 @@ const char* rfmf_version(void);
 /* ---------------------------------------------------------------------- */
 
 /* Dense CSR view: rows sorted by neighbour id, each undirected edge stored
- * twice.  Replaces anneallib::AnnealCsr as seen by the kernel
- * (src/immutable_graph.hpp:154-157, src/anneal_gpu.cuh:23-28). */
+ * twice.  Replaces anneallib::AnnealCsr (src/immutable_graph.hpp:154-157)
+ * as the kernel sees it through DevCsr (src/anneal_gpu.cuh:23-28). */
 typedef struct rfmf_csr {
     const uint32_t* offsets;      /* n + 1 */
     const int32_t* edge_to;       /* m2 */
@@ -68,7 +68,8 @@ typedef struct rfmf_read_stats {
 typedef struct rfmf_graph rfmf_graph;
 
 /* read_instance_from_file + removeDegreeZeroVertices + AnnealCsr
- * (src/read_gset.hpp:32-34, src/host_common.cpp:316-327,463-475). */
+ * (src/read_gset.hpp:32-34, src/host_common.cpp:184-194,
+ * src/immutable_graph.hpp:154-157). */
 rfmf_status rfmf_graph_read_gset(const char* path, double forced_scale, rfmf_graph** out,
                                  rfmf_read_stats* stats);
 /* Graph::addEdge over an edge list in order (first weight wins, zero weights
@@ -93,7 +94,7 @@ int32_t rfmf_beta_fixed_for_step(uint64_t step, double t0);
 /* annealTotalSteps (src/host_common.h:67-68). */
 rfmf_status rfmf_total_steps(const uint64_t* iters, int64_t n, int64_t* total);
 /* Rng(raw seed) + generateRandomSpins (src/rng.hpp:11-58,
- * src/host_common.cpp:171-178). */
+ * src/host_common.cpp:40-45). */
 rfmf_status rfmf_initial_spins(uint64_t raw_seed, int64_t n, int8_t* out);
 
 /* Run-length beta tables: run r of a table covers 0-based steps

@@ -93,7 +93,7 @@ int32_t or_vertex_gumbel(uint64_t solver_seed, uint64_t step, uint64_t v) {
 
 /* ------------------------------------------------------------------------ */
 /* Initial spins -- src/rng.hpp:11-58 (splitmix64 seeding of xoshiro256++,   */
-/* Vigna) and src/host_common.cpp:171-178 (one bit per vertex, +1 if set).   */
+/* Vigna) and src/host_common.cpp:40-45 (one bit per vertex, +1 if set).     */
 /* ------------------------------------------------------------------------ */
 
 uint64_t or_splitmix64_next(uint64_t* state) {
@@ -133,7 +133,8 @@ void or_initial_spins(uint64_t raw_seed, int64_t n, int8_t* spin_out) {
 /* ------------------------------------------------------------------------ */
 /* Beta schedule -- src/host_common.h:73-76 (value), :82-94 (run layout).    */
 /* The runs are found here by a plain per-step scan, not by the reference's  */
-/* bisection (host_common.cpp:347-406), so the two are independent.          */
+/* gallop and bisection (firstStepAbove, appendBetaRuns:                    */
+/* src/host_common.cpp:212-273), so the two are independent.                 */
 /* ------------------------------------------------------------------------ */
 
 int32_t or_beta_fixed(uint64_t step, double t0) {
@@ -198,7 +199,7 @@ int64_t or_beta_runs_linear(int32_t b0, int32_t b1, int64_t total,
 }
 
 /* ------------------------------------------------------------------------ */
-/* int32 gate -- src/host_common.cpp:429-445; B -- :447-458.                 */
+/* int32 gate -- src/host_common.cpp:296-312; B -- :314-325.                 */
 /* ------------------------------------------------------------------------ */
 
 #define OR_GUMBEL_MAX_MAG 3617613LL   /* LUT[15] + 127 (philox.cuh:44-48) */
@@ -240,7 +241,8 @@ int64_t or_cut_value(const uint32_t* off, const int32_t* to, const int32_t* w,
 /* CSR build -- src/graph.cpp:11-17 (dedup, first weight wins),              */
 /* src/immutable_graph.hpp:44-116 (dense ids by ascending original id, rows  */
 /* sorted by dense neighbour id).  Vertices are exactly the edge endpoints,  */
-/* so degree-zero removal (host_common.cpp:316-327) is implied.              */
+/* so degree-zero removal (removeDegreeZeroVertices,                         */
+/* src/host_common.cpp:184-194) is implied.                                  */
 /* Implemented with sorting instead of hash maps.                            */
 /* ------------------------------------------------------------------------ */
 

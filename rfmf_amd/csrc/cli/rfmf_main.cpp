@@ -1,5 +1,7 @@
 // rfmf_main -- command-line driver with the reference binary's interface
-// (main.cu:20-103, src/host_common.cpp:237-314,460-576): the same flags, the
+// (main.cu:20-103; printUsage, parseArgs and setupSolver at
+// src/host_common.cpp:73-181,327-387, printChainResults and printSummary at
+// src/host_common.cpp:389-443): the same flags, the
 // same stdout blocks (device line, two partition lines per chain, summary)
 // and the same stderr lines the reference's shell tests parse ("Resolved
 // seeds:", "chain i: kernel-best-val: ... kernel-final-val: ...").  Every
@@ -57,11 +59,12 @@ Args parse(int argc, char** argv) {
     bool hs = false, hp = false, ht = false, hi = false;
     for (int i = 2; i < argc; i += 2) {
         const std::string flag = argv[i];
-        if (i + 1 >= argc) { std::cerr << "Error: missing value for " << flag << "\n"; std::exit(1); }
+        if (i + 1 >= argc) { std::cerr << "rfmf_main: option " << flag << " needs a value\n"; std::exit(1); }
         const std::string val = argv[i + 1];
         try {
             if (flag == "--seed" || flag == "--t0") {
-                std::cerr << "Error: one chain per (seed, t0) pair; use --seeds/--t0s (zipped lists).\n";
+                std::cerr << "rfmf_main: " << flag << " is not accepted: every chain is one (seed, t0) pair, "
+                             "given as --seeds a,b,... with --t0s x,y,... (lists of equal length)\n";
                 std::exit(1);
             } else if (flag == "--seeds") {
                 a.seeds = parseList<uint64_t>(val, [](const std::string& t) { return std::stoull(t); });
@@ -78,35 +81,38 @@ Args parse(int argc, char** argv) {
             } else if (flag == "--device") {
                 a.device = std::stoi(val);
             } else {
-                std::cerr << "Error: unknown option '" << flag << "'\n";
+                std::cerr << "rfmf_main: unrecognized option " << flag << "\n";
                 usage(argv[0]);
                 std::exit(1);
             }
         } catch (const std::exception&) {
-            std::cerr << "Error: bad value for " << flag << ": " << val << "\n";
+            std::cerr << "rfmf_main: cannot parse the value of " << flag << ": '" << val << "'\n";
             std::exit(1);
         }
     }
+    // Every failed check is reported, then the usage text once.
+    const auto count = [](size_t k) { return std::to_string(k); };
+    const std::pair<bool, std::string> checks[] = {
+        {!hs, "--seeds missing (required)"},
+        {!hp, "--phases missing (required)"},
+        {!ht, "--t0s missing (required)"},
+        {!hi, "--iters missing (required)"},
+        {ht && a.t0s.empty(), "--t0s lists no temperature"},
+        {ht && std::any_of(a.t0s.begin(), a.t0s.end(), [](double t) { return !(t > 0.0); }),
+         "--t0s entries must all be positive"},
+        {hs && ht && a.seeds.size() != a.t0s.size(),
+         "--seeds has " + count(a.seeds.size()) + " entries but --t0s has " + count(a.t0s.size()) +
+             "; chain i uses seed i with t0 i"},
+        {hp && hi && a.iters.size() != a.phases.size(),
+         "--iters has " + count(a.iters.size()) + " entries but --phases has " + count(a.phases.size()) +
+             "; phase p runs iters[p] sweeps with P = phases[p] groups"},
+    };
     bool ok = true;
-    if (!hs) { std::cerr << "Error: --seeds is required\n"; ok = false; }
-    if (!hp) { std::cerr << "Error: --phases is required\n"; ok = false; }
-    if (!ht) { std::cerr << "Error: --t0s is required\n"; ok = false; }
-    if (!hi) { std::cerr << "Error: --iters is required\n"; ok = false; }
-    if (ht && a.t0s.empty()) { std::cerr << "Error: --t0s must have at least one entry\n"; ok = false; }
-    if (ht && std::any_of(a.t0s.begin(), a.t0s.end(), [](double t) { return !(t > 0.0); })) {
-        std::cerr << "Error: every --t0s entry must be > 0\n";
-        ok = false;
-    }
-    if (hs && ht && a.seeds.size() != a.t0s.size()) {
-        std::cerr << "Error: --seeds must have the same number of entries as --t0s (" << a.t0s.size()
-                  << "); the lists are zipped, one chain per pair\n";
-        ok = false;
-    }
-    if (hp && hi && a.iters.size() != a.phases.size()) {
-        std::cerr << "Error: --iters must have the same number of elements as --phases (" << a.phases.size()
-                  << ")\n";
-        ok = false;
-    }
+    for (const auto& c : checks)
+        if (c.first) {
+            std::cerr << "rfmf_main: " << c.second << "\n";
+            ok = false;
+        }
     if (!ok) { usage(argv[0]); std::exit(1); }
     return a;
 }

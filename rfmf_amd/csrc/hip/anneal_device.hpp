@@ -3,7 +3,8 @@
 //
 // HBM layout (all chains of all graphs in one address space):
 //   graph   offsets u32[n+1]; edges int2[2m] = {dense neighbour id, weight},
-//           rows sorted by neighbour id (the reference's AnnealCsr order,
+//           rows sorted by neighbour id (the order the reference's CSR
+//           build sorts each row's edgeTo_/edgeWeight_ slice into,
 //           src/immutable_graph.hpp:83-101).  Read-only, shared by every chain
 //           of the graph.
 //   chain   words   : packed per-vertex state, one word per vertex

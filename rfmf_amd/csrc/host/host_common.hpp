@@ -1,7 +1,9 @@
 // host_common.hpp -- host-side inputs of the annealing path: beta schedules,
 // the int32 score gate, the weighted-degree bound, degree-zero removal and
-// the initial spins (reference: src/host_common.h:22-131,
-// src/host_common.cpp:171-178,316-458).  Errors are reported as rfmf::Error
+// the initial spins (reference: src/host_common.h:22-131; generateRandomSpins,
+// removeDegreeZeroVertices, annealTotalSteps, buildBetaRunTables,
+// annealScoreSafeInt32 and maxWeightedDegree at
+// src/host_common.cpp:40-45,184-325).  Errors are reported as rfmf::Error
 // (a status code and a message) instead of terminating the process.
 #pragma once
 
@@ -95,18 +97,18 @@ int32_t linearBetaForStep(uint64_t step, int64_t totalSteps, int32_t b0, int32_t
 
 // True iff the packed int32 score path is safe: round(ln(1+T)/t0 * 2^20) * B +
 // |Gumbel|max < INT32_MAX, with the product guarded against int64 overflow
-// (src/host_common.cpp:429-445).
+// (src/host_common.cpp:296-312).
 bool annealScoreSafeInt32(int64_t totalSteps, double t0, int64_t deltaBound);
 // Same verdict from an explicit largest |beta| (used for arbitrary tables).
 bool scoreSafeInt32ForBeta(int64_t betaMaxFixed, int64_t deltaBound);
 
-// B = max_v sum |w| over v's row (src/host_common.cpp:447-458).
+// B = max_v sum |w| over v's row (src/host_common.cpp:314-325).
 int64_t maxWeightedDegree(const anneallib::AnnealCsr& ig);
 
 // Removes every degree-zero vertex in place; must run before AnnealCsr(g)
-// because it changes the dense id space.
+// because it changes the dense id space (src/host_common.cpp:184-194).
 void removeDegreeZeroVertices(anneallib::Graph& g);
 
 // spin[i] = +1 if the next reservoir bit is set, else -1, i = 0..n-1, from the
-// generator seeded with the RAW seed (src/host_common.cpp:171-178).
+// generator seeded with the RAW seed (src/host_common.cpp:40-45).
 std::vector<int8_t> generateRandomSpins(uint64_t n, anneallib::Rng& rng);

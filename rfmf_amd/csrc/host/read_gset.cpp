@@ -48,19 +48,34 @@ double pickScale(double maxB, double maxW, double maxDeg, double budgetB) {
     return 0.0;
 }
 
+// "u v w": two integer endpoints and a weight (integer or decimal), each
+// field required; anything after the weight is ignored.
+bool parseEdgeLine(const std::string& line, RawEdge& e) {
+    const char* p = line.c_str();
+    char* next = nullptr;
+    e.u = std::strtoll(p, &next, 10);
+    bool ok = next != p;
+    p = next;
+    e.v = std::strtoll(p, &next, 10);
+    ok = ok && next != p;
+    p = next;
+    e.w = std::strtod(p, &next);
+    return ok && next != p;
+}
+
 }  // namespace
 
 bool read_instance_from_stream(std::istream& in, Graph& g, ReadStats* stats, double forcedScale) {
     std::string line;
     if (!readDataLine(in, line)) {
-        std::cerr << "Error: no header line (numVertices numEdges)\n";
+        std::cerr << "read_gset: missing header line '<vertices> <edges>'\n";
         return false;
     }
     int64_t headerN = 0, headerM = 0;
     {
         std::istringstream hs(line);
         if (!(hs >> headerN >> headerM) || headerN < 0 || headerM < 0) {
-            std::cerr << "Error: failed to read header (numVertices numEdges)\n";
+            std::cerr << "read_gset: header is not two non-negative integers: '" << line << "'\n";
             return false;
         }
     }
@@ -70,20 +85,14 @@ bool read_instance_from_stream(std::istream& in, Graph& g, ReadStats* stats, dou
     bool anyFloat = false;
     for (int64_t i = 0; i < headerM; ++i) {
         if (!readDataLine(in, line)) {
-            std::cerr << "Error: failed to read edge " << i << "\n";
+            std::cerr << "read_gset: the file ends after " << i << " of the header's " << headerM << " edges\n";
             return false;
         }
-        const char* p = line.c_str();
-        char* end = nullptr;
         RawEdge e{};
-        e.u = std::strtoll(p, &end, 10);
-        if (end == p) { std::cerr << "Error: bad edge line " << i << "\n"; return false; }
-        p = end;
-        e.v = std::strtoll(p, &end, 10);
-        if (end == p) { std::cerr << "Error: bad edge line " << i << "\n"; return false; }
-        p = end;
-        e.w = std::strtod(p, &end);
-        if (end == p) { std::cerr << "Error: bad edge line " << i << "\n"; return false; }
+        if (!parseEdgeLine(line, e)) {
+            std::cerr << "read_gset: edge " << i << " is not '<u> <v> <weight>': '" << line << "'\n";
+            return false;
+        }
         if (!integralWeight(e.w)) anyFloat = true;
         raw.push_back(e);
     }
@@ -114,13 +123,13 @@ bool read_instance_from_stream(std::istream& in, Graph& g, ReadStats* stats, dou
         scale = pickScale(maxB, maxW, maxDeg, static_cast<double>(INT32_MAX) / 4);
         if (scale == 0.0) scale = pickScale(maxB, maxW, maxDeg, static_cast<double>(INT64_MAX) / 8);
         if (scale == 0.0) {
-            std::cerr << "Error: no power-of-10 scale fits these weights into int32 weights / "
-                         "int64 deltas (max|w|=" << maxW << ", maxB=" << maxB << ")\n";
+            std::cerr << "read_gset: weights up to " << maxW << " (weighted degree up to " << maxB
+                      << ") fit no power-of-ten scale with int32 weights and int64 fields\n";
             return false;
         }
         if (maxW * scale < 0.5) {
-            std::cerr << "Error: chosen scale " << scale << " rounds every weight to 0 (max|w|="
-                      << maxW << ")\n";
+            std::cerr << "read_gset: scale " << scale << " rounds every weight (largest |w| " << maxW
+                      << ") to zero\n";
             return false;
         }
     }
@@ -154,7 +163,7 @@ bool read_instance_from_file(const std::string& filename, Graph& g, ReadStats* s
                              double forcedScale) {
     std::ifstream f(filename);
     if (!f.is_open()) {
-        std::cerr << "Error: could not open file: " << filename << "\n";
+        std::cerr << "read_gset: cannot open " << filename << "\n";
         return false;
     }
     return read_instance_from_stream(f, g, stats, forcedScale);

@@ -234,11 +234,12 @@ CLI = ROOT / "rfmf_amd" / "bin" / "rfmf_main"
 
 @pytest.mark.parametrize("args,needle", [
     ([], "Usage"),
-    (["x", "--seed", "1"], "one chain per (seed, t0) pair"),
-    (["x", "--seeds", "1,2", "--t0s", "1", "--phases", "1", "--iters", "1"], "same number of entries"),
-    (["x", "--seeds", "1", "--t0s", "0", "--phases", "1", "--iters", "1"], "must be > 0"),
-    (["x", "--seeds", "1", "--t0s", "1", "--phases", "1,2", "--iters", "1"], "same number of elements"),
-    (["x", "--bogus", "1"], "unknown option"),
+    (["x", "--seed", "1"], "one (seed, t0) pair"),
+    (["x", "--seeds", "1,2", "--t0s", "1", "--phases", "1", "--iters", "1"], "--seeds has 2 entries but --t0s has 1"),
+    (["x", "--seeds", "1", "--t0s", "0", "--phases", "1", "--iters", "1"], "must all be positive"),
+    (["x", "--seeds", "1", "--t0s", "1", "--phases", "1,2", "--iters", "1"], "--iters has 1 entries but --phases has 2"),
+    (["x", "--bogus", "1"], "unrecognized option"),
+    (["x", "--seeds", "1"], "--t0s missing (required)"),
 ])
 def test_cli_argument_errors(args, needle):
     r = subprocess.run([str(CLI), *args], capture_output=True, text=True)

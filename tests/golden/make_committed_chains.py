@@ -95,7 +95,12 @@ def main():
         runs[run] = {}
         for inst in insts:
             chains = parse(run, inst)
+            err = (REF / run / "logs" / f"{inst}.err").read_text().splitlines()
+            out = (REF / run / "logs" / f"{inst}.out").read_text().splitlines()
+            heads = [l for l in err if l.startswith(("Loaded:", "Resolved seeds:", "Chains:"))]
+            heads += [l for l in out if l.startswith("summary:")]
             runs[run][inst] = {
+                "log_lines": heads,
                 "phases": chains[0]["phases"], "iters": chains[0]["iters"], "n": chains[0]["n"],
                 "chains": [[c["seed"], c["t0"], c["best_cut"], c["final_cut"], c["kernel_best"], c["kernel_final"],
                             short_digest(c["best_partition"]), short_digest(c["final_partition"])]

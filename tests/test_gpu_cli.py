@@ -92,3 +92,55 @@ def test_chain_isolation(label, inst, seeds, t0s, phases, iters, idx):
 ])
 def test_determinism(inst, seeds, t0s, phases, iters):
     assert run(inst, seeds, t0s, phases, iters) == run(inst, seeds, t0s, phases, iters)
+
+
+# ------------------------------------------------------- committed benchmark logs
+
+import json  # noqa: E402
+
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def _committed(inst):
+    fx = json.loads((GOLDEN / "committed_digests.json").read_text())
+    rec = fx["runs"]["V100-SXM2-10M"][inst]
+    return [dict(zip(fx["fields"], c)) for c in rec["chains"]], rec
+
+
+@pytest.mark.parametrize("inst", ["G62", "G63", "G70", "G81"])
+def test_cli_lines_match_committed_logs(inst):
+    """The reference's own benchmark command for the instance (its 80 resolved
+    seeds and t0s, ladder 128..1) with 100 sweeps per phase: the stderr
+    banner lines equal the committed run's (results/V100-SXM2-10M/logs/*.err)
+    except the sweep total, and every stdout line has the committed .out
+    layout: the same device-line prefix per chain, partitions of n bits and
+    the summary fields."""
+    chains, rec = _committed(inst)
+    seeds = ",".join(str(c["seed"]) for c in chains)
+    t0s = ",".join(repr(c["t0"]) for c in chains)
+    phases = ",".join(str(p) for p in rec["phases"])
+    iters = ",".join(["100"] * len(rec["phases"]))
+    args = [str(CLI), str(GSET / inst), "--seeds", seeds, "--t0s", t0s, "--phases", phases, "--iters", iters]
+    print(f"[cli-log] {inst}: 80 committed chains x 800 sweeps", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    loaded, resolved, banner, summary = rec["log_lines"]
+    err = r.stderr.splitlines()
+    assert loaded in err and resolved in err
+    total = sum(rec["iters"])
+    assert banner.replace(f"total-sweeps-per-chain={total}", "total-sweeps-per-chain=800") in err
+    out = r.stdout.splitlines()
+    n = rec["n"]
+    assert len(out) == 3 * len(chains) + 1
+    for c, ch in enumerate(chains):
+        head = (f"device: GPU instance: {inst} --seed {ch['seed']} --phases {phases} --t0 {ch['t0']:.4f} "
+                f"--iters {iters} best-cut-value: ")
+        line = out[3 * c]
+        assert line.startswith(head), (line[:200], head)
+        assert re.fullmatch(r"-?\d+ final-cut-value: -?\d+ time-sec: \d+\.\d{4}", line[len(head):]), line
+        assert re.fullmatch(r"Solution-best-partition: [01]{%d}" % n, out[3 * c + 1])
+        assert re.fullmatch(r"Solution-final-partition: [01]{%d}" % n, out[3 * c + 2])
+    assert re.fullmatch(r"summary: chains: 80 best-chain: \d+ best-cut-value: -?\d+ time-sec: \d+\.\d{4} "
+                        r"chains-at-best: \d+", out[-1]), out[-1]
+    assert re.fullmatch(r"summary: chains: 80 best-chain: \d+ best-cut-value: -?\d+ time-sec: \d+\.\d{4} "
+                        r"chains-at-best: \d+", summary)

@@ -135,3 +135,50 @@ def test_replay_every_committed_chain_of_a_run(run_name):
     print(f"[replay-run] {run_name}: {nchains - len(bad)}/{nchains} chains byte-identical to the committed "
           f"partitions; mismatching: {bad[:40]}", flush=True)
     assert not bad
+
+
+@pytest.mark.skipif(os.environ.get("RFMF_LONG_CLI") != "1", reason="80 chains x 8e7 sweeps; RFMF_LONG_CLI=1")
+def test_cli_reproduces_the_committed_g62_log():
+    """The reference's committed command for G62 (results/V100-SXM2-10M/
+    reproduce.sh: 80 seeds and t0s, ladder 128..1, 10^7 sweeps per phase) run
+    through this build's CLI: every stdout line equals the committed G62.out
+    line (rebuilt from the fixtures: the device lines, both partitions of all
+    80 chains, the summary) with time-sec masked, and the stderr banner and
+    per-chain kernel values equal the committed G62.err lines."""
+    import re
+    import subprocess
+    fx = json.loads((GOLDEN / "committed_digests.json").read_text())
+    rec = fx["runs"]["V100-SXM2-10M"]["G62"]
+    full = {c["chain"]: c for c in json.loads((GOLDEN / "committed_chains.json").read_text())["chains"]
+            if c["instance"] == "G62" and c["run"] == "V100-SXM2-10M"}
+    assert sorted(full) == list(range(80))
+    chains = [full[c] for c in range(80)]
+    phases = ",".join(str(p) for p in rec["phases"])
+    iters = ",".join(str(i) for i in rec["iters"])
+    expect = []
+    for c in chains:
+        expect.append(f"device: GPU instance: G62 --seed {c['seed']} --phases {phases} --t0 {c['t0']:.4f} "
+                      f"--iters {iters} best-cut-value: {c['best_cut']} final-cut-value: {c['final_cut']} "
+                      f"time-sec: <t>")
+        for key, label in (("best_partition", "best"), ("final_partition", "final")):
+            bits = np.unpackbits(np.frombuffer(bytes.fromhex(c[key]), np.uint8))[:c["n"]]
+            expect.append(f"Solution-{label}-partition: " + "".join("1" if b else "0" for b in bits))
+    expect.append(rec["log_lines"][3])
+    cli = GOLDEN.parent.parent / "rfmf_amd" / "bin" / "rfmf_main"
+    args = [str(cli), str(GSET / "G62"), "--seeds", ",".join(str(c["seed"]) for c in chains),
+            "--t0s", ",".join(repr(c["t0"]) for c in chains), "--phases", phases, "--iters", iters]
+    print("[cli-replay] G62: the committed command, 80 chains x 8e7 sweeps", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=1000)
+    assert r.returncode == 0, r.stderr[-2000:]
+    mask = lambda s: re.sub(r"time-sec: [0-9.]+", "time-sec: <t>", s)
+    got = [mask(l) for l in r.stdout.splitlines()]
+    want = [mask(l) for l in expect]
+    bad = [i for i, (a, b) in enumerate(zip(got, want)) if a != b]
+    print(f"[cli-replay] G62: {len(got)} stdout lines, {len(want)} expected, {len(bad)} differ "
+          f"(first: {bad[:5]})", flush=True)
+    assert len(got) == len(want) and not bad
+    err = r.stderr.splitlines()
+    for line in rec["log_lines"][:3]:
+        assert line in err, line
+    for c, ch in enumerate(chains):
+        assert f"chain {c}: kernel-best-val: {ch['kernel_best']} kernel-final-val: {ch['kernel_final']}" in err

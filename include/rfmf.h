@@ -342,8 +342,10 @@ typedef struct rfmf_server_stats {
 
 /* A server owns a worker thread that keeps two slices of `slice_sweeps`
  * sweeps over every admitted chain queued on the device, admits queued
- * requests (FIFO) while their HBM footprint fits `hbm_budget_bytes`, and
- * retires finished requests while the next slice runs. */
+ * requests in arrival order while their HBM footprint fits
+ * `hbm_budget_bytes` (a request that does not fit yet is passed by at most 8
+ * later requests that do, then blocks the queue until it fits), and retires
+ * finished requests while the next slice runs. */
 rfmf_status rfmf_server_create(rfmf_ctx* ctx, int64_t hbm_budget_bytes, int64_t slice_sweeps,
                                rfmf_server** out);
 /* Queues a request (graph copied, inputs copied); outputs are written into

@@ -3,7 +3,8 @@
 // A worker thread owns the device loop and keeps two slices queued on the
 // context's stream: every admitted chain advances by `slice` sweeps per slice,
 // in one persistent launch per kernel variant.  While the older slice runs,
-// the loop admits queued requests whose HBM footprint fits the budget (CSR
+// the loop admits queued requests whose HBM footprint fits the budget, in
+// arrival order with a bounded bypass of requests that do not fit yet (CSR
 // upload, chain-state allocation and the initial-word kernel on the side
 // stream, all from a stream-ordered pool, so nothing waits for the running
 // kernels), and once it has finished, retires the requests whose last sweeps
@@ -46,6 +47,7 @@ struct Request {
     rfmf::JobSpec spec;
     int64_t footprint = 0;
     double tSubmit = -1, tAdmit = -1, tDone = -1;
+    int bypassed = 0;         // later requests admitted while this one waited
     bool done = false;
     bool failed = false;
     std::string error;
@@ -75,15 +77,34 @@ struct rfmf_server {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
 
-    // Pops every queued request that fits the remaining budget (FIFO; a
-    // request that does not fit yet blocks the ones behind it, so arrival
-    // order is kept).
+    // A queued request that does not fit yet can be passed by at most this
+    // many later requests that do; then nothing passes it until it fits.
+    static constexpr int kMaxBypass = 8;
+
+    // Pops the queued requests that fit the remaining budget, in arrival
+    // order.  A request that does not fit stays queued and the scan goes on
+    // behind it (small requests need not wait for a large one to fit), but
+    // each admission past it counts against its bypass allowance; a request
+    // whose allowance is spent blocks everything behind it, so it cannot
+    // starve.
     std::vector<std::shared_ptr<Request>> takeAdmissible() {
         std::vector<std::shared_ptr<Request>> out;
-        while (!queue.empty() && resident + queue.front()->footprint <= budget) {
-            resident += queue.front()->footprint;
-            out.push_back(queue.front());
-            queue.pop_front();
+        std::vector<Request*> waiting;   // requests ahead of the scan that did not fit
+        for (auto it = queue.begin(); it != queue.end();) {
+            Request& r = **it;
+            if (resident + r.footprint <= budget) {
+                resident += r.footprint;
+                for (Request* w : waiting) w->bypassed += 1;
+                out.push_back(*it);
+                it = queue.erase(it);
+                continue;
+            }
+            if (r.bypassed >= kMaxBypass) break;
+            waiting.push_back(&r);
+            bool spent = false;
+            for (Request* w : waiting) spent = spent || w->bypassed >= kMaxBypass;
+            if (spent) break;
+            ++it;
         }
         return out;
     }

@@ -287,3 +287,34 @@ def test_score_table_lean_and_rich_builds():
     end, best, cv, bv = run_oracle_chains(csr, init, solver, beta, phases, iters)
     assert np.array_equal(small.end_spins[:8], end) and np.array_equal(small.best_spins[:8], best)
     assert np.array_equal(small.final_vals[:8], cv) and np.array_equal(small.best_vals[:8], bv)
+
+
+def test_server_admission_passes_a_blocked_request_at_most_eight_times(graphs):
+    # A long request holds most of the budget; a large request behind it
+    # cannot fit until it retires; ten small requests arrive after the large
+    # one and fit beside the long one.  The first eight pass the large
+    # request, the last two wait behind it; every result equals its one-shot
+    # run.
+    long_req = make_job(graphs["torus"], list(range(601, 801)), LADDER, [12500] * 8)
+    large = make_job(graphs["torus"], list(range(801, 1101)), LADDER, [10] * 8)
+    small = [make_job(graphs["G11"], [1200 + i], [8, 1], [25, 25]) for i in range(10)]
+    fp = lambda j: R.footprint_bytes(j.csr.n, j.csr.m2, j.chains, j.beta.run_end.size, len(j.phases))
+    budget = fp(long_req) + fp(large) - 1
+    assert fp(long_req) + 10 * fp(small[0]) <= budget and fp(large) <= budget
+    print(f"[serve-bypass] budget {budget} B: long {fp(long_req)} B, large {fp(large)} B, small {fp(small[0])} B",
+          flush=True)
+    with R.Context(0) as ctx:
+        with R.Server(ctx, budget, slice_sweeps=500) as srv:
+            ids = [srv.submit(long_req), srv.submit(large)] + [srv.submit(j) for j in small]
+            for rid in ids:
+                srv.wait(rid, 120.0)
+            t = [srv.poll(rid) for rid in ids]
+    t_long_done, t_large_admit = t[0][3], t[1][2]
+    admits = [x[2] for x in t[2:]]
+    print(f"[serve-bypass] long done {t_long_done:.4f}, large admitted {t_large_admit:.4f}, small admitted "
+          f"{[round(a, 4) for a in admits]}", flush=True)
+    assert t_large_admit >= t_long_done
+    assert all(a < t_large_admit for a in admits[:8])
+    assert all(a >= t_large_admit for a in admits[8:])
+    for j in [long_req, large] + small:
+        same(j, one_shot(j), "bypass")

@@ -119,6 +119,7 @@ def main() -> int:
     ap.add_argument("--qps", type=float, default=120.0, help="serve: Poisson arrival rate")
     ap.add_argument("--requests", type=int, default=96, help="serve: number of requests")
     ap.add_argument("--slice", type=int, default=500, help="serve: sweeps per serving slice")
+    ap.add_argument("--no-plain", action="store_true", help="serve: served leg only (no plain FIFO comparison)")
     ap.add_argument("--graphs", type=int, default=256, help="batch: number of graphs")
     ap.add_argument("--batch-sweeps", type=int, default=20000, help="batch: sweeps per chain")
     ap.add_argument("--gset-sweeps", type=int, default=100000, help="gset: sweeps per chain")
@@ -421,7 +422,8 @@ def bench_batch(args) -> int:
     return 0 if r["identical_results"] else 1
 
 
-def run_serve(R, nreq: int, qps: float, slice_sweeps: int, lo: int = 10_000, hi: int = 100_000) -> dict:
+def run_serve(R, nreq: int, qps: float, slice_sweeps: int, lo: int = 10_000, hi: int = 100_000,
+              plain_leg: bool = True) -> dict:
     """Serving: the request stream into the continuous-batching server (HBM
     footprint admission), against plain per-request launches served one at a
     time in arrival order (upload, one-graph launch, download per request).
@@ -443,7 +445,7 @@ def run_serve(R, nreq: int, qps: float, slice_sweeps: int, lo: int = 10_000, hi:
         # plain: one request at a time, first come first served
         t0 = time.perf_counter()
         finish = []
-        for i, j in enumerate(plain):
+        for i, j in enumerate(plain if plain_leg else []):
             wait = arrivals[i] - (time.perf_counter() - t0)
             if wait > 0:
                 time.sleep(wait)
@@ -451,9 +453,12 @@ def run_serve(R, nreq: int, qps: float, slice_sweeps: int, lo: int = 10_000, hi:
             ctx.batch_run([(gid, j)])
             ctx.release_graph(gid)
             finish.append(time.perf_counter() - t0)
-        lat_p = np.array(finish) - arrivals
-        span_p = finish[-1] - arrivals[0]
-        log(f"[serve] plain FIFO done in {finish[-1]:.1f} s")
+        if plain_leg:
+            lat_p = np.array(finish) - arrivals
+            span_p = finish[-1] - arrivals[0]
+            log(f"[serve] plain FIFO done in {finish[-1]:.1f} s")
+        else:
+            lat_p, span_p = np.full(nreq, np.nan), float("nan")
         with R.Server(ctx, hbm_budget_bytes=64 << 30, slice_sweeps=slice_sweeps) as srv:
             t0 = time.perf_counter()
             ids = []
@@ -468,8 +473,9 @@ def run_serve(R, nreq: int, qps: float, slice_sweeps: int, lo: int = 10_000, hi:
             st = srv.stats()
     lat_s = np.array([td - ts for _, ts, _, td in polls])
     span_s = max(td for *_, td in polls) - min(ts for _, ts, _, _ in polls)
-    same = all(np.array_equal(a.end_spins, b.end_spins) and np.array_equal(a.best_spins, b.best_spins) and
-               np.array_equal(a.best_vals, b.best_vals) for a, b in zip(served, plain))
+    same = plain_leg and all(np.array_equal(a.end_spins, b.end_spins) and
+                             np.array_equal(a.best_spins, b.best_spins) and
+                             np.array_equal(a.best_vals, b.best_vals) for a, b in zip(served, plain))
     med_s = float(np.median(np.concatenate([j.best_vals for j in served])))
     med_p = float(np.median(np.concatenate([j.best_vals for j in plain])))
     out = {"config": f"serve_{nreq}req_{qps:g}qps_sweeps{lo}-{hi}", "served_value": work / span_s,
@@ -485,9 +491,9 @@ def run_serve(R, nreq: int, qps: float, slice_sweeps: int, lo: int = 10_000, hi:
 
 def bench_serve(args) -> int:
     import rfmf_amd as R
-    r = run_serve(R, args.requests, args.qps, args.slice)
+    r = run_serve(R, args.requests, args.qps, args.slice, plain_leg=not args.no_plain)
     _emit({"metric": "chain-steps/sec and solve latency", **r})
-    return 0 if r["identical_results"] else 1
+    return 0 if r["identical_results"] or args.no_plain else 1
 
 
 def bench_gset(args) -> int:

@@ -91,20 +91,20 @@ struct rfmf_server {
         std::vector<std::shared_ptr<Request>> out;
         std::vector<Request*> waiting;   // requests ahead of the scan that did not fit
         for (auto it = queue.begin(); it != queue.end();) {
+            // Nothing passes a waiting request whose allowance is spent.
+            bool spent = false;
+            for (const Request* w : waiting) spent = spent || w->bypassed >= kMaxBypass;
+            if (spent) break;
             Request& r = **it;
             if (resident + r.footprint <= budget) {
                 resident += r.footprint;
                 for (Request* w : waiting) w->bypassed += 1;
                 out.push_back(*it);
                 it = queue.erase(it);
-                continue;
+            } else {
+                waiting.push_back(&r);
+                ++it;
             }
-            if (r.bypassed >= kMaxBypass) break;
-            waiting.push_back(&r);
-            bool spent = false;
-            for (Request* w : waiting) spent = spent || w->bypassed >= kMaxBypass;
-            if (spent) break;
-            ++it;
         }
         return out;
     }

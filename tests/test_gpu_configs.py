@@ -154,3 +154,24 @@ def test_poisson_request_stream_served_matches_oracle():
     for i, j in enumerate(jobs):
         oracle_check(j, range(j.chains), f"request {i} (n={j.csr.n}, {j.chains} chains)")
     print(f"[config5] every chain of every request equals the oracle ({time.time() - t:.1f} s)", flush=True)
+
+
+@pytest.mark.parametrize("name,n,m,pm1", [("sparse_n10000_m10000_unit", 10000, 10000, False),
+                                           ("sparse_n7000_m17150_pm1", 7000, 17150, True)])
+def test_sparse_graphs_with_4096_chains_match_oracle(name, n, m, pm1):
+    """The sparse configuration's graphs with 4096 chains in one persistent
+    launch (autotuned geometry): every chain's cuts recomputed on the host, a
+    sample of chains against the oracle."""
+    csr = W.sparse_random(n, m, pm1, seed=0)
+    sweeps = 400
+    job = make_job(csr, range(1, 4097), [sweeps // 8] * 8)
+    print(f"[config3] {name}: n={csr.n} (after degree-zero removal), 4096 chains x {sweeps} sweeps", flush=True)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ctx.batch_run([(gid, job)], autotune=True)
+        geo = ctx.last_launch()
+    print(f"[config3] {name}: {geo}", flush=True)
+    for c in range(4096):
+        assert R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c]
+        assert R.cut_value_csr(csr, job.end_spins[c]) == job.final_vals[c]
+    oracle_check(job, list(range(0, 4096, 128)) + [4095], name)

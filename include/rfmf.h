@@ -235,18 +235,23 @@ typedef struct rfmf_tuning {
     int32_t state_in_hbm;       /* 0: a chain keeps its state in LDS for a slice when it fits there
                                    and works on it in place in HBM otherwise; 1: every chain in
                                    place in HBM; output-invariant */
+    int32_t rows_in_lds;        /* graphs whose weights are all +1/-1 (n <= 32768): 0 copy the
+                                   graph's offsets and 2-byte rows into LDS beside the chain when
+                                   that keeps as many chains resident, 1 whenever they fit, -1
+                                   never; output-invariant */
 } rfmf_tuning;
 
 /* Geometry of the context's last batch launch: threads per chain, grid,
  * state word bits, resident workgroups per CU, the autotuner's trial ms per
  * sweep for 128/256/512/1024 threads (0 where not tried), the score
  * arithmetic (0 int32, 1 int32 with 24-bit multiplies, 2 int64, 3 24-bit with
- * the LDS score table) and whether the chain state stayed in HBM (1) or was
- * LDS-resident (0).  Any output pointer may be NULL.  Replaces the
- * reference's hard-coded V100 launch constants (src/tuning.hpp:8-18). */
+ * the LDS score table), whether the chain state stayed in HBM (1) or was
+ * LDS-resident (0), and whether the graph's rows were read from LDS (1).  Any
+ * output pointer may be NULL.  Replaces the reference's hard-coded V100
+ * launch constants (src/tuning.hpp:8-18). */
 rfmf_status rfmf_ctx_last_launch(rfmf_ctx* ctx, int32_t* threads, int32_t* grid, int32_t* word_bits,
                                  int32_t* occupancy, float* trial_ms_per_sweep, int32_t* score_mode,
-                                 int32_t* state_in_hbm);
+                                 int32_t* state_in_hbm, int32_t* rows_in_lds);
 
 /* Runs every job's chains to completion in persistent launches that pack all
  * jobs together; returns the kernel-only time of all launches. */

@@ -323,9 +323,9 @@ class Job:
 
 
 def _tuning(slice_sweeps=0, grid_cap=0, use_hip_graph=False, threads=0, autotune=False,
-            score_table=0, state_in_hbm=False) -> _capi.Tuning:
+            score_table=0, state_in_hbm=False, rows_in_lds=0) -> _capi.Tuning:
     return _capi.Tuning(int(slice_sweeps), int(grid_cap), int(bool(use_hip_graph)), int(threads),
-                        int(bool(autotune)), int(score_table), int(bool(state_in_hbm)))
+                        int(bool(autotune)), int(score_table), int(bool(state_in_hbm)), int(rows_in_lds))
 
 
 class Context:
@@ -383,27 +383,30 @@ class Context:
 
     def batch_run(self, jobs: list[tuple[int, Job]], slice_sweeps: int = 0, grid_cap: int = 0,
                   use_hip_graph: bool = False, threads: int = 0, autotune: bool = False,
-                  score_table: int = 0, state_in_hbm: bool = False) -> float:
+                  score_table: int = 0, state_in_hbm: bool = False, rows_in_lds: int = 0) -> float:
         """Runs (graph_id, Job) pairs to completion in packed persistent launches.
         score_table (8-bit state words): 0 auto, 1 the LDS score table whenever
         it fits, -1 never.  state_in_hbm: every chain works on its state in
-        place in HBM (by default only chains too large for LDS do).  Results
-        are identical either way."""
+        place in HBM (by default only chains too large for LDS do).
+        rows_in_lds (graphs with +/-1 weights, n <= 32768): 0 auto, 1 the
+        graph's rows in LDS whenever they fit, -1 never.  Results are
+        identical either way."""
         arr = (_capi.Job * len(jobs))(*[j.struct(gid) for gid, j in jobs])
-        tun = _tuning(slice_sweeps, grid_cap, use_hip_graph, threads, autotune, score_table, state_in_hbm)
+        tun = _tuning(slice_sweeps, grid_cap, use_hip_graph, threads, autotune, score_table, state_in_hbm,
+                      rows_in_lds)
         ms = C.c_float()
         check(lib().rfmf_batch_run(self._h, arr, len(jobs), C.byref(tun), C.byref(ms)))
         return ms.value
 
     def last_launch(self) -> dict:
         """Geometry of the last batch launch and the autotuner's trial times."""
-        t, g, b, o, m, h = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        t, g, b, o, m, h, r = (C.c_int32() for _ in range(7))
         trial = (C.c_float * 4)()
         check(lib().rfmf_ctx_last_launch(self._h, C.byref(t), C.byref(g), C.byref(b), C.byref(o), trial,
-                                         C.byref(m), C.byref(h)))
+                                         C.byref(m), C.byref(h), C.byref(r)))
         return {"threads_per_chain": t.value, "grid": g.value, "word_bits": b.value, "wg_per_cu": o.value,
                 "score_mode": ("int32", "int32-mul24", "int64", "mul24-table")[m.value],
-                "state_in_hbm": bool(h.value),
+                "state_in_hbm": bool(h.value), "rows_in_lds": bool(r.value),
                 "trial_ms_per_sweep": {n: trial[i] for i, n in enumerate((128, 256, 512, 1024)) if trial[i] > 0}}
 
     def tune_flips(self, graph_id: int, sweeps: int, top_ps, beta=(0.1, 3.0), pilot_chains: int = 0,
@@ -448,8 +451,9 @@ class Run:
         check(lib().rfmf_run_create(ctx.handle, self._arr, len(jobs), C.byref(self._h)))
 
     def advance(self, sweeps: int, use_hip_graph: bool = False, grid_cap: int = 0,
-                threads: int = 0, score_table: int = 0, state_in_hbm: bool = False) -> tuple[float, bool]:
-        tun = _tuning(0, grid_cap, use_hip_graph, threads, False, score_table, state_in_hbm)
+                threads: int = 0, score_table: int = 0, state_in_hbm: bool = False,
+                rows_in_lds: int = 0) -> tuple[float, bool]:
+        tun = _tuning(0, grid_cap, use_hip_graph, threads, False, score_table, state_in_hbm, rows_in_lds)
         ms = C.c_float()
         fin = C.c_int32()
         check(lib().rfmf_run_advance(self._h, int(sweeps), C.byref(tun), C.byref(ms), C.byref(fin)))

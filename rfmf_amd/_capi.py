@@ -47,7 +47,7 @@ class Job(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("slice_sweeps", C.c_int64), ("grid_cap", C.c_int32), ("use_hip_graph", C.c_int32),
                 ("threads_per_chain", C.c_int32), ("autotune", C.c_int32), ("score_table", C.c_int32),
-                ("state_in_hbm", C.c_int32)]
+                ("state_in_hbm", C.c_int32), ("rows_in_lds", C.c_int32)]
 
 
 MAX_FLIPS_CANDIDATES = 16
@@ -102,7 +102,7 @@ SIGNATURES = {
     "rfmf_ctx_release_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "rfmf_ctx_device_info": (C.c_int, [C.c_void_p, i32p, i64p, i64p, C.c_char_p, C.c_int32]),
     "rfmf_ctx_copy_probe": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, f32p]),
-    "rfmf_ctx_last_launch": (C.c_int, [C.c_void_p, i32p, i32p, i32p, i32p, f32p, i32p, i32p]),
+    "rfmf_ctx_last_launch": (C.c_int, [C.c_void_p, i32p, i32p, i32p, i32p, f32p, i32p, i32p, i32p]),
     "rfmf_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int64, C.POINTER(Tuning), f32p]),
     "rfmf_batch_run_windows": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_int64, i64p, C.c_int64,
                                          C.POINTER(Tuning), f32p]),

@@ -6,7 +6,11 @@
 //           rows sorted by neighbour id (the order the reference's CSR
 //           build sorts each row's edgeTo_/edgeWeight_ slice into,
 //           src/immutable_graph.hpp:83-101).  Read-only, shared by every chain
-//           of the graph.
+//           of the graph.  offsets are padded to 16 B.
+//           Graphs whose weights are all +1 or -1 and whose n is at most 32768
+//           also carry compact rows u16[2m] (neighbour id | 0x8000 for weight
+//           -1, padded to 16 B), which launches with LDS to spare copy into
+//           LDS beside the offsets.
 //   chain   words   : packed per-vertex state, one word per vertex
 //                     (spin bit | delta + bias, delta = spin*S = the cut gain
 //                     of flipping the vertex), padded to 16 B.
@@ -18,7 +22,8 @@
 //           end, group count P) per chain.
 //
 // LDS layout (per workgroup = one chain at a time): words, bestbits, the
-// per-wave group partials, the resolved winners, the Gumbel LUT.
+// per-wave group partials, the resolved winners, the Gumbel LUT, and when the
+// plan has room, the graph's offsets and compact rows.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -53,8 +58,15 @@ struct alignas(16) ChainItem {
     int32_t nRuns;
     int32_t nPhases;
     int32_t deltaMax;          // bound on |delta| (max weighted degree, capped at 2^31 - 1)
+    const uint16_t* rows16;    // compact rows (nullptr when the graph has none)
+    uint32_t m2;               // directed edges (2m)
+    int32_t pad;
 };
-static_assert(sizeof(ChainItem) == 112, "ChainItem layout");
+static_assert(sizeof(ChainItem) == 128, "ChainItem layout");
+
+// Compact row entry: neighbour id in bits 0-14, bit 15 set for weight -1.
+constexpr int64_t kCompactMaxN = 32768;
+constexpr uint32_t kCompactNeg = 0x8000u;
 
 // Word encodings: spin bit at the top of the word, delta = spin*S biased to
 // stay non-negative so that 32-bit LDS atomic adds of signed changes never

@@ -524,7 +524,12 @@ constexpr int kSliceWavesPerEu =
 // (graphs whose state does not fit LDS).  FLIP's global atomics complete in
 // L2, so after the sweep's last barrier each wave invalidates its L1 (an
 // agent-scope acquire) before the next sweep reads the words.
-template <int kBits, int kMode, int NT, bool kHbm, bool kRich = false>
+// kRows: the build that copies the graph's offsets and compact rows into LDS
+// for items that have them (LdsPlan::rowsOn), for launches of about one chain
+// per CU; only the register-rich 2- and 4-wave tiles and the 8- and 16-wave
+// tiles have it (launch::rowsCapable), and the builds without it are the same
+// code as before the path existed.
+template <int kBits, int kMode, int NT, bool kHbm, bool kRich = false, bool kRows = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kSliceWavesPerEu<kBits, kMode, NT, kHbm, kRich>)))
 void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __restrict__ sched, LdsPlan plan,
                  uint64_t sliceSweeps) {
@@ -535,6 +540,8 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     constexpr bool kMul24 = kMode == kScore24 || kTab;
     constexpr bool kWide = kMode == kScore64;
     constexpr bool kPackable = !kWide && NT <= 256;   // packed per-lane argmax keys
+    static_assert(!kRows || (!kHbm && (kRich || NT >= 512)),
+                  "rows in LDS: state in LDS, register-rich 2- and 4-wave tiles or 8- and 16-wave tiles");
     static_assert(!kTab || kBits == 8, "the score table needs 8-bit words (field < 128)");
     using Score = std::conditional_t<kWide, int64_t, int32_t>;
     constexpr Score kNoScore = kWide ? static_cast<Score>(INT64_MIN) : static_cast<Score>(INT32_MIN);
@@ -635,6 +642,23 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
             for (int i = tid; i < bq; i += NT) lb[i] = gb[i];
         }
         uint8_t* const bl8 = reinterpret_cast<uint8_t*>(bl);
+        // ---- the graph's rows: copied into LDS when the plan has room and
+        // the graph has compact rows (FLIP's row walk is then a chain of LDS
+        // reads instead of dependent L2 loads) ----
+        const uint32_t* const rowOffL = reinterpret_cast<const uint32_t*>(lds + plan.rowsOff);
+        const uint16_t* const rowEdgL = reinterpret_cast<const uint16_t*>(lds + plan.rowEdgesOff);
+        const bool rowsInLds =
+            kRows && plan.rowsOn != 0 && uni(static_cast<int32_t>(it.rows16 != nullptr)) != 0;
+        if (rowsInLds) {
+            const int rq = (n + 4) >> 2;   // u32x4 pieces of offsets[0..n] (padded to 16 B)
+            const int eq = static_cast<int>((uni(it.m2) + 7u) >> 3);   // u32x4 pieces of the u16 rows
+            const GlobalOf<u32x4>* go = globalIn(reinterpret_cast<const u32x4*>(it.offsets));
+            u32x4* lo = reinterpret_cast<u32x4*>(lds + plan.rowsOff);
+            for (int i = tid; i < rq; i += NT) lo[i] = go[i];
+            const GlobalOf<u32x4>* ge = globalIn(reinterpret_cast<const u32x4*>(it.rows16));
+            u32x4* le = reinterpret_cast<u32x4*>(lds + plan.rowEdgesOff);
+            for (int i = tid; i < eq; i += NT) le[i] = ge[i];
+        }
         int64_t cur = uni(st->cur);
         int64_t best = uni(st->best);
         uint64_t step = uni(st->step);
@@ -1074,9 +1098,9 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                     int32_t b = -1, sb = 0;
                     if (g < P) { b = pk[g]; sb = ps[g]; }
                     Delta own = 0;   // sum over adjacent winners x of 2 w s_b s_x
-                    if (b >= 0) {
-                        const uint32_t lo = offsets[b];
-                        const uint32_t hi = offsets[b + 1];
+                    // Walks row [lo, hi) of winner b; edgeAt(i) gives entry i
+                    // as {neighbour, weight}.
+                    auto walk = [&](uint32_t lo, uint32_t hi, auto edgeAt) {
                         // Two edges per step: both rows entries are fetched,
                         // then both neighbours' group slots and words are
                         // read, then the updates are applied.  A non-winner's
@@ -1086,8 +1110,8 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                         for (uint32_t i0 = lo + sub; i0 < hi; i0 += 2 * L) {
                             const uint32_t i1 = i0 + static_cast<uint32_t>(L);
                             const bool two = i1 < hi;
-                            const i32x2 e0 = edges[i0];
-                            const i32x2 e1 = two ? edges[i1] : i32x2{e0.x, 0};
+                            const i32x2 e0 = edgeAt(i0);
+                            const i32x2 e1 = two ? edgeAt(i1) : i32x2{e0.x, 0};
                             const int gx0 = groupOf(e0.x, pRecip);
                             const int gx1 = groupOf(e1.x, pRecip);
                             const int32_t pk0 = pk[gx0];
@@ -1110,6 +1134,16 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                             };
                             apply(e0.x, Delta{2} * e0.y, gx0, pk0, wd0);
                             if (two) apply(e1.x, Delta{2} * e1.y, gx1, pk1, wd1);
+                        }
+                    };
+                    if (b >= 0) {
+                        if (rowsInLds) {
+                            walk(rowOffL[b], rowOffL[b + 1], [&](uint32_t i) {
+                                const uint32_t c = rowEdgL[i];
+                                return i32x2{static_cast<int32_t>(c & ~kCompactNeg), (c & kCompactNeg) ? -1 : 1};
+                            });
+                        } else {
+                            walk(offsets[b], offsets[b + 1], [&](uint32_t i) { return static_cast<i32x2>(edges[i]); });
                         }
                     }
                     // Sum over the winner's L lanes (an aligned lane group).
@@ -1208,13 +1242,22 @@ hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s) {
 }
 
 // The compiled variants: word width x score mode x threads per chain x
-// state in LDS or HBM.
+// state in LDS or HBM, the register-rich builds, and the rows-in-LDS builds.
 template <int kBits, int kMode, bool kHbm>
-static const void* kernelFor(int threads, bool rich) {
+static const void* kernelFor(int threads, bool rich, bool rows) {
     if constexpr ((kMode == dev::kScoreTab || kMode == dev::kScore24) && !kHbm) {
-        if (rich && threads == 128) return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 128, kHbm, true>);
-        if (rich && threads == 256) return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 256, kHbm, true>);
+        if (rich && threads == 128)
+            return rows ? reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 128, kHbm, true, true>)
+                        : reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 128, kHbm, true>);
+        if (rich && threads == 256)
+            return rows ? reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 256, kHbm, true, true>)
+                        : reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 256, kHbm, true>);
     }
+    if constexpr (!kHbm) {
+        if (rows && threads == 512) return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 512, kHbm, false, true>);
+        if (rows && threads == 1024) return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 1024, kHbm, false, true>);
+    }
+    if (rows) return nullptr;
     switch (threads) {
         case 128: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 128, kHbm>);
         case 256: return reinterpret_cast<const void*>(dev::annealSlice<kBits, kMode, 256, kHbm>);
@@ -1225,35 +1268,36 @@ static const void* kernelFor(int threads, bool rich) {
 }
 
 template <int kBits, bool kHbm>
-static const void* kernelForMode(int mode, int threads, bool rich) {
+static const void* kernelForMode(int mode, int threads, bool rich, bool rows) {
     switch (mode) {
         case dev::kScore32:   // int32 scores up to 32-bit words
-            if constexpr (kBits <= 32) return kernelFor<kBits, dev::kScore32, kHbm>(threads, false);
+            if constexpr (kBits <= 32) return kernelFor<kBits, dev::kScore32, kHbm>(threads, false, rows);
             return nullptr;
         case dev::kScore24:   // 24-bit fields only (8/16-bit words)
-            if constexpr (kBits <= 16) return kernelFor<kBits, dev::kScore24, kHbm>(threads, rich);
+            if constexpr (kBits <= 16) return kernelFor<kBits, dev::kScore24, kHbm>(threads, rich, rows);
             return nullptr;
-        case dev::kScore64: return kernelFor<kBits, dev::kScore64, kHbm>(threads, false);
+        case dev::kScore64: return kernelFor<kBits, dev::kScore64, kHbm>(threads, false, rows);
         case dev::kScoreTab:   // 8-bit words with LDS-resident state only
-            if constexpr (kBits == 8 && !kHbm) return kernelFor<kBits, dev::kScoreTab, kHbm>(threads, rich);
+            if constexpr (kBits == 8 && !kHbm) return kernelFor<kBits, dev::kScoreTab, kHbm>(threads, rich, rows);
             return nullptr;
         default: return nullptr;
     }
 }
 
-static const void* kernelFor(int bits, int mode, int threads, bool hbm, bool rich) {
+static const void* kernelFor(int bits, int mode, int threads, bool hbm, bool rich, bool rows) {
     switch (bits) {
-        case 8:  return hbm ? kernelForMode<8, true>(mode, threads, rich) : kernelForMode<8, false>(mode, threads, rich);
-        case 16: return hbm ? kernelForMode<16, true>(mode, threads, rich) : kernelForMode<16, false>(mode, threads, rich);
-        case 32: return hbm ? kernelForMode<32, true>(mode, threads, rich) : kernelForMode<32, false>(mode, threads, rich);
-        case 64: return hbm ? kernelForMode<64, true>(mode, threads, rich) : kernelForMode<64, false>(mode, threads, rich);
+        case 8:  return hbm ? kernelForMode<8, true>(mode, threads, rich, rows) : kernelForMode<8, false>(mode, threads, rich, rows);
+        case 16: return hbm ? kernelForMode<16, true>(mode, threads, rich, rows) : kernelForMode<16, false>(mode, threads, rich, rows);
+        case 32: return hbm ? kernelForMode<32, true>(mode, threads, rich, rows) : kernelForMode<32, false>(mode, threads, rich, rows);
+        case 64: return hbm ? kernelForMode<64, true>(mode, threads, rich, rows) : kernelForMode<64, false>(mode, threads, rich, rows);
         default: return nullptr;
     }
 }
 
-hipError_t anneal(int bits, int mode, int threads, bool hbm, bool rich, const ChainItem* items, int nItems,
-                  uint32_t* sched, const LdsPlan& plan, int grid, uint64_t sliceSweeps, hipStream_t s) {
-    const void* kern = kernelFor(bits, mode, threads, hbm, rich);
+hipError_t anneal(int bits, int mode, int threads, bool hbm, bool rich, bool rows, const ChainItem* items,
+                  int nItems, uint32_t* sched, const LdsPlan& plan, int grid, uint64_t sliceSweeps, hipStream_t s) {
+    if (rows != (plan.rowsOn != 0)) return hipErrorInvalidValue;
+    const void* kern = kernelFor(bits, mode, threads, hbm, rich, rows);
     if (!kern) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        static_cast<int>(plan.total));
@@ -1263,8 +1307,8 @@ hipError_t anneal(int bits, int mode, int threads, bool hbm, bool rich, const Ch
     return hipLaunchKernel(kern, dim3(grid), dim3(threads), args, plan.total, s);
 }
 
-int maxActiveBlocks(int bits, int mode, int threads, bool hbm, bool rich, size_t lds) {
-    const void* kern = kernelFor(bits, mode, threads, hbm, rich);
+int maxActiveBlocks(int bits, int mode, int threads, bool hbm, bool rich, bool rows, size_t lds) {
+    const void* kern = kernelFor(bits, mode, threads, hbm, rich, rows);
     if (!kern) return 0;
     if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) !=
         hipSuccess)

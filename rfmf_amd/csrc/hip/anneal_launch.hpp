@@ -47,9 +47,12 @@ struct LdsPlan {
     uint32_t deltaOff;    // int32 winner pre-flip delta per group
     uint32_t dpartOff;    // int64 per-wave cut-delta partials
     uint32_t miscOff;     // scheduler broadcast
+    uint32_t rowsOff;     // graph offsets u32[n+1] (rowsOn only)
+    uint32_t rowEdgesOff; // compact rows u16[2m] (rowsOn only)
     uint32_t total;       // dynamic LDS bytes
     int32_t partStride;   // u64 entries per wave in the partial table
     int32_t slotWords;    // u64 entries of the partial/slot region
+    int32_t rowsOn;       // items with compact rows copy the graph into LDS
 };
 
 // One chain's initial-state build.
@@ -66,7 +69,12 @@ struct InitItem {
 inline uint32_t alignUp(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
 
 // hbm: the chain state stays in HBM, so the plan has no words or best bits.
-inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads, int mode, bool hbm = false) {
+// rowsM2 >= 0: room for a graph of maxN vertices and rowsM2 directed edges in
+// compact form (offsets and u16 rows), which the kernel copies into LDS for
+// every item that has compact rows.
+inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads, int mode, bool hbm = false,
+                           int64_t rowsM2 = -1) {
+    const int64_t rowsN = maxN;
     LdsPlan p{};
     if (hbm) maxN = 0;
     const int waves = threads / 64;
@@ -83,6 +91,11 @@ inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads, int mo
     p.deltaOff = off; off += alignUp(static_cast<uint32_t>(maxP) * (bits == 64 ? 8u : 4u), 16);
     p.dpartOff = off; off += alignUp(static_cast<uint32_t>(waves) * 8u, 16);
     p.miscOff = off;  off += 16;
+    if (rowsM2 >= 0) {
+        p.rowsOn = 1;
+        p.rowsOff = off;     off += alignUp(static_cast<uint32_t>(rowsN + 1) * 4u, 16);
+        p.rowEdgesOff = off; off += alignUp(static_cast<uint32_t>(rowsM2) * 2u, 16);
+    }
     p.total = off;
     p.partStride = laneP;
     p.slotWords = slots;
@@ -99,13 +112,18 @@ namespace launch {
 // `hbm`: the chains' state stays in HBM (no LDS copy).  `rich`: the 24-bit
 // score variants' 2- and 4-wave tiles (state in LDS) built for 5 resident
 // workgroups per CU instead of 6 (more registers for the scoring loop's
-// schedule); ignored elsewhere.
-hipError_t anneal(int bits, int mode, int threads, bool hbm, bool rich, const dev::ChainItem* items, int nItems,
-                  uint32_t* sched, const dev::LdsPlan& plan, int grid, uint64_t sliceSweeps,
+// schedule); ignored elsewhere.  `rows`: the rows-in-LDS build
+// (rowsCapable variants only; must equal plan.rowsOn).
+hipError_t anneal(int bits, int mode, int threads, bool hbm, bool rich, bool rows, const dev::ChainItem* items,
+                  int nItems, uint32_t* sched, const dev::LdsPlan& plan, int grid, uint64_t sliceSweeps,
                   hipStream_t s);
 
+// Whether a variant carries the rows-in-LDS path (LdsPlan::rowsOn): state in
+// LDS, and the register-rich build or 8- and 16-wave tiles.
+inline bool rowsCapable(int threads, bool hbm, bool rich) { return !hbm && (rich || threads >= 512); }
+
 // Resident workgroups per CU for a variant at a given dynamic LDS size.
-int maxActiveBlocks(int bits, int mode, int threads, bool hbm, bool rich, size_t lds);
+int maxActiveBlocks(int bits, int mode, int threads, bool hbm, bool rich, bool rows, size_t lds);
 
 // Device-to-device copy of `bytes` (a multiple of 16) with a plain kernel.
 hipError_t streamCopy(const void* src, void* dst, size_t bytes, hipStream_t s);

@@ -362,7 +362,7 @@ rfmf_status rfmf_ctx_device_info(rfmf_ctx* ctx, int32_t* cu_count, int64_t* lds_
 
 rfmf_status rfmf_ctx_last_launch(rfmf_ctx* ctx, int32_t* threads, int32_t* grid, int32_t* word_bits,
                                  int32_t* occupancy, float* trial_ms_per_sweep, int32_t* score_mode,
-                                 int32_t* state_in_hbm) {
+                                 int32_t* state_in_hbm, int32_t* rows_in_lds) {
     return guarded([&] {
         if (!ctx) throw Error(Status::InvalidArg, "null context");
         std::lock_guard<std::mutex> lk(ctx->ctx->mutex());
@@ -373,6 +373,7 @@ rfmf_status rfmf_ctx_last_launch(rfmf_ctx* ctx, int32_t* threads, int32_t* grid,
         if (occupancy) *occupancy = r.occupancy;
         if (score_mode) *score_mode = r.mode;
         if (state_in_hbm) *state_in_hbm = r.hbm ? 1 : 0;
+        if (rows_in_lds) *rows_in_lds = r.rows ? 1 : 0;
         if (trial_ms_per_sweep)
             for (int i = 0; i < 4; ++i) trial_ms_per_sweep[i] = r.trialMsPerSweep[i];
     });
@@ -449,6 +450,9 @@ rfmf::LaunchKnobs knobsFrom(const rfmf_tuning* t) {
         if (t->state_in_hbm < 0 || t->state_in_hbm > 1)
             throw Error(Status::InvalidArg, "state_in_hbm must be 0 or 1");
         k.stateInHbm = t->state_in_hbm;
+        if (t->rows_in_lds < -1 || t->rows_in_lds > 1)
+            throw Error(Status::InvalidArg, "rows_in_lds must be -1, 0 or 1");
+        k.rowsInLds = t->rows_in_lds;
     }
     return k;
 }

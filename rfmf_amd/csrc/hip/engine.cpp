@@ -9,6 +9,8 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <unordered_set>
@@ -85,6 +87,22 @@ Context::Context(int device) : device_(device) {
     for (hipEvent_t& e : join_) hipCheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     sched_.alloc(2 * kMaxGroups * sizeof(uint32_t));
     hipCheck(hipMemset(sched_.get(), 0, sched_.bytes()), "hipMemset");
+    const char* pr = std::getenv("RFMF_PROGRESS");
+    progress_ = pr && *pr && std::strcmp(pr, "0") != 0;
+}
+
+void Context::reportProgress(const std::vector<ResidentJob*>& jobs, float kernelMs) const {
+    uint64_t done = ~uint64_t{0}, total = 0;
+    size_t chains = 0;
+    for (const ResidentJob* j : jobs) {
+        total = std::max(total, j->totalSteps);
+        chains += j->stepsDone.size();
+        for (uint64_t s : j->stepsDone) done = std::min(done, s);
+    }
+    if (chains == 0) return;
+    std::fprintf(stderr, "rfmf: %zu chains at sweep %llu of %llu, %.1f s of kernel time\n", chains,
+                 static_cast<unsigned long long>(done), static_cast<unsigned long long>(total), kernelMs / 1e3);
+    std::fflush(stderr);
 }
 
 Context::~Context() {
@@ -163,6 +181,17 @@ int32_t Context::uploadGraph(const uint32_t* offsets, const int32_t* to, const i
     return uploadGraph(prepareGraph(offsets, to, w, n, m2), async);
 }
 
+std::vector<uint16_t> compactRows(const HostGraph& g) {
+    std::vector<uint16_t> rows;
+    if (g.m2 <= 0 || g.n > dev::kCompactMaxN) return rows;
+    for (int32_t w : g.w)
+        if (w != 1 && w != -1) return rows;
+    rows.assign(static_cast<size_t>((g.m2 + 7) / 8 * 8), 0);
+    for (int64_t e = 0; e < g.m2; ++e)
+        rows[e] = static_cast<uint16_t>(static_cast<uint32_t>(g.to[e]) | (g.w[e] < 0 ? dev::kCompactNeg : 0u));
+    return rows;
+}
+
 int32_t Context::uploadGraph(HostGraph&& hg, hipStream_t async) {
     auto g = std::make_unique<DevGraph>();
     g->n = hg.n;
@@ -171,24 +200,35 @@ int32_t Context::uploadGraph(HostGraph&& hg, hipStream_t async) {
     g->host = std::move(hg);
     const int64_t n = g->n, m2 = g->m2;
     hipCheck(hipSetDevice(device_), "hipSetDevice");
+    // Offsets padded to 16 B (kernels copy them into LDS in 16-byte pieces).
     const size_t offBytes = static_cast<size_t>(n + 1) * sizeof(uint32_t);
+    const size_t offAlloc = (offBytes + 15) / 16 * 16;
     const size_t edgeBytes = static_cast<size_t>(std::max<int64_t>(m2, 1)) * 8;
+    const std::vector<uint16_t> rows = compactRows(g->host);
+    const size_t rowBytes = rows.size() * sizeof(uint16_t);
     if (async) {
-        g->offsets.allocAsync(offBytes, pool(), async);
+        g->offsets.allocAsync(offAlloc, pool(), async);
         g->edges.allocAsync(edgeBytes, pool(), async);
+        if (rowBytes) g->rows16.allocAsync(rowBytes, pool(), async);
         hipCheck(hipMemcpyAsync(g->offsets.get(), g->host.offsets.data(), offBytes, hipMemcpyHostToDevice, async),
                  "upload offsets");
         if (m2 > 0)
             hipCheck(hipMemcpyAsync(g->edges.get(), g->host.edges.data(), m2 * 8, hipMemcpyHostToDevice, async),
                      "upload edges");
+        if (rowBytes)
+            hipCheck(hipMemcpyAsync(g->rows16.get(), rows.data(), rowBytes, hipMemcpyHostToDevice, async),
+                     "upload compact rows");
         hipCheck(hipStreamSynchronize(async), "upload graph");
     } else {
-        g->offsets.alloc(offBytes);
+        g->offsets.alloc(offAlloc);
         g->edges.alloc(edgeBytes);
+        if (rowBytes) g->rows16.alloc(rowBytes);
         hipCheck(hipMemcpy(g->offsets.get(), g->host.offsets.data(), offBytes, hipMemcpyHostToDevice),
                  "upload offsets");
         if (m2 > 0)
             hipCheck(hipMemcpy(g->edges.get(), g->host.edges.data(), m2 * 8, hipMemcpyHostToDevice), "upload edges");
+        if (rowBytes)
+            hipCheck(hipMemcpy(g->rows16.get(), rows.data(), rowBytes, hipMemcpyHostToDevice), "upload compact rows");
     }
     std::vector<int32_t>().swap(g->host.edges);
     const int32_t id = nextGraph_++;
@@ -209,13 +249,14 @@ const DevGraph* Context::graph(int32_t id) const {
     return it->second.get();
 }
 
-int Context::occupancy(int bits, int mode, int threads, size_t lds, bool hbm, bool rich) {
-    const uint64_t key = (static_cast<uint64_t>(rich) << 57) | (static_cast<uint64_t>(hbm) << 56) |
+int Context::occupancy(int bits, int mode, int threads, size_t lds, bool hbm, bool rich, bool rows) {
+    const uint64_t key = (static_cast<uint64_t>(rows) << 58) | (static_cast<uint64_t>(rich) << 57) |
+                         (static_cast<uint64_t>(hbm) << 56) |
                          (static_cast<uint64_t>(bits) << 48) |
                          (static_cast<uint64_t>(mode) << 44) | (static_cast<uint64_t>(threads) << 32) | lds;
     const auto it = occCache_.find(key);
     if (it != occCache_.end()) return it->second;
-    const int occ = launch::maxActiveBlocks(bits, mode, threads, hbm, rich, lds);
+    const int occ = launch::maxActiveBlocks(bits, mode, threads, hbm, rich, rows, lds);
     occCache_[key] = occ;
     return occ;
 }
@@ -461,6 +502,8 @@ std::unique_ptr<ResidentJob> Context::placeJob(const HostJob& hj, const DevGraph
         it.stepEnd = hj.totalSteps;
         it.n = static_cast<int32_t>(n);
         it.deltaMax = static_cast<int32_t>(std::min<int64_t>(g->B, INT32_MAX));
+        it.rows16 = g->rows16.bytes() ? g->rows16.as<uint16_t>() : nullptr;
+        it.m2 = static_cast<uint32_t>(g->m2);
     }
     return job;
 }
@@ -487,15 +530,26 @@ void Context::finishGroup(LaunchGroup& g, int64_t maxN, int maxP, const LaunchKn
     g.threads = knobs.threads > 0 ? knobs.threads
                 : tuned > 0      ? tuned
                                  : chooseThreads(g.bits, g.mode, maxN, nItems, maxP, g.hbm);
-    shapeGroup(this, props_, g, maxN, nItems, knobs);
+    int64_t maxM2 = 0;
+    for (const dev::ChainItem& it : g.items) {
+        if (!it.rows16) { maxM2 = -1; break; }
+        maxM2 = std::max<int64_t>(maxM2, it.m2);
+    }
+    shapeGroup(this, props_, g, maxN, maxM2, nItems, knobs);
 }
 
-void shapeGroup(Context* ctx, const hipDeviceProp_t& props, GroupShape& g, int64_t maxN, int64_t nItems,
+namespace {
+// Directed edges beyond which compact rows are never planned into LDS (their
+// 2 B per edge alone would exceed any workgroup's LDS).
+constexpr int64_t kMaxLdsRowsM2 = int64_t{1} << 17;
+}  // namespace
+
+void shapeGroup(Context* ctx, const hipDeviceProp_t& props, GroupShape& g, int64_t maxN, int64_t maxM2, int64_t nItems,
                 const LaunchKnobs& knobs) {
     // LDS plan, score arithmetic and grid for one launch group.
-    auto occupancy = [&](int bits, int mode, int threads, size_t lds, bool hbm, bool rich) {
-        return ctx ? ctx->occupancy(bits, mode, threads, lds, hbm, rich)
-                   : launch::maxActiveBlocks(bits, mode, threads, hbm, rich, lds);
+    auto occupancy = [&](int bits, int mode, int threads, size_t lds, bool hbm, bool rich, bool rows = false) {
+        return ctx ? ctx->occupancy(bits, mode, threads, lds, hbm, rich, rows)
+                   : launch::maxActiveBlocks(bits, mode, threads, hbm, rich, rows, lds);
     };
     if (!dev::validThreads(g.threads))
         throw Error(Status::InvalidArg, "threads per chain must be 128, 256, 512 or 1024");
@@ -532,6 +586,23 @@ void shapeGroup(Context* ctx, const hipDeviceProp_t& props, GroupShape& g, int64
         if (occRich > 0 && residentOf(occRich) >= residentOf(occ)) {
             g.rich = true;
             occ = occRich;
+        }
+    }
+    // The graph's offsets and compact rows in LDS, for FLIP's row walk: taken
+    // when they keep every chain resident that the plan without them would
+    // (in practice launches of one or two chains per CU, whose sweeps are
+    // latency-bound).
+    g.rows = false;
+    if (maxM2 >= 0 && maxM2 <= kMaxLdsRowsM2 && launch::rowsCapable(g.threads, g.hbm, g.rich) &&
+        knobs.rowsInLds >= 0) {
+        const dev::LdsPlan rp = dev::makeLdsPlan(maxN, g.bits, g.maxP, g.threads, g.mode, false, maxM2);
+        if (static_cast<size_t>(rp.total) + dev::staticLds(g.mode) <= props.sharedMemPerBlock) {
+            const int occRows = occupancy(g.bits, g.mode, g.threads, rp.total, false, g.rich, true);
+            if (occRows > 0 && (knobs.rowsInLds > 0 || residentOf(occRows) >= residentOf(occ))) {
+                g.plan = rp;
+                g.rows = true;
+                occ = occRows;
+            }
         }
     }
     g.occupancy = occ;
@@ -629,6 +700,7 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
         last_.occupancy = g.occupancy;
         last_.mode = g.mode;
         last_.hbm = g.hbm;
+        last_.rows = g.rows;
     }
     return out;
 }
@@ -648,7 +720,7 @@ hipError_t Context::enqueueGroups(const std::vector<LaunchGroup>& groups,
     for (size_t i = 0; i < groups.size() && err == hipSuccess; ++i) {
         const LaunchGroup& g = groups[i];
         const size_t l = i % kLanes;
-        keep(launch::anneal(g.bits, g.mode, g.threads, g.hbm, g.rich, items[i], static_cast<int>(g.items.size()),
+        keep(launch::anneal(g.bits, g.mode, g.threads, g.hbm, g.rich, g.rows, items[i], static_cast<int>(g.items.size()),
                             sched_.as<uint32_t>() + 2 * i, g.plan, g.grid, slice,
                             l == 0 ? stream_ : lanes_[l - 1]));
     }
@@ -722,7 +794,7 @@ std::unique_ptr<Context::GraphedSlices> Context::captureSlices(const std::vector
         g->itemBufs.emplace_back(lg.items.size() * sizeof(dev::ChainItem));
         hipCheck(hipMemcpy(g->itemBufs.back().get(), lg.items.data(), g->itemBufs.back().bytes(),
                            hipMemcpyHostToDevice), "upload items");
-        (void)launch::maxActiveBlocks(lg.bits, lg.mode, lg.threads, lg.hbm, lg.rich, lg.plan.total);   // LDS attribute, outside the capture
+        (void)launch::maxActiveBlocks(lg.bits, lg.mode, lg.threads, lg.hbm, lg.rich, lg.rows, lg.plan.total);   // LDS attribute, outside the capture
     }
     hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -888,6 +960,7 @@ float Context::runUntil(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
             while (lowest(at) && at < end) {
                 k.sliceSweeps = sliceTo(at, end);
                 total += runSlice(jobs, k);
+                if (progress_) reportProgress(jobs, total);
             }
         }
         return total;
@@ -910,6 +983,7 @@ float Context::runUntil(const std::vector<ResidentJob*>& jobs, const LaunchKnobs
         LaunchKnobs k = knobs;
         k.sliceSweeps = sliceTo(at, until);
         total += runSlice(jobs, k);
+        if (progress_) reportProgress(jobs, total);
     }
     return total;
 }

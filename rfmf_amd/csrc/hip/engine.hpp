@@ -85,10 +85,15 @@ struct HostGraph {
 // Validates a CSR (offsets monotone from 0 to m2, neighbour ids in range)
 // and builds its HostGraph.  Host-only; any thread.
 HostGraph prepareGraph(const uint32_t* offsets, const int32_t* to, const int32_t* w, int64_t n, int64_t m2);
+// Compact rows of a graph whose weights are all +1 or -1 and whose n is at
+// most 32768: one u16 per directed edge (neighbour id | 0x8000 for weight -1),
+// padded to a multiple of 8 entries; empty for any other graph.
+std::vector<uint16_t> compactRows(const HostGraph& g);
 
 struct DevGraph {
-    DevBuf offsets;   // u32[n+1]
+    DevBuf offsets;   // u32[n+1], padded to 16 B
     DevBuf edges;     // int2[m2] {to, weight}
+    DevBuf rows16;    // compact rows u16[m2] padded to 16 B (empty when the graph has none)
     int64_t n = 0;
     int64_t m2 = 0;
     int64_t B = 0;    // max weighted degree
@@ -161,6 +166,7 @@ struct LaunchKnobs {
     int threads = 0;           // threads per chain (128/256/512/1024); 0: tuned or heuristic
     bool autotune = false;     // time the candidates on the run's first slices
     int scoreTable = 0;        // 8-bit words: 0 table when it costs no resident chains, 1 always, -1 never
+    int rowsInLds = 0;         // compact rows in LDS: 0 when they cost no resident chains, 1 whenever they fit, -1 never
     int stateInHbm = 0;        // 0: only chains whose state exceeds LDS keep it in HBM; 1: every chain
     // Tiles the autotuner chose per launch group (Context::groupKey); a group
     // not listed uses `threads`, else the cached choice or the heuristic.
@@ -179,6 +185,7 @@ struct GroupShape {
     int maxP = 1;
     bool hbm = false;
     bool rich = false;   // 24-bit-score build for 5 workgroups per CU (launch::anneal)
+    bool rows = false;   // plan.rowsOn: graphs with compact rows read them from LDS
 };
 
 class Context;
@@ -192,10 +199,12 @@ bool fitsLds(const hipDeviceProp_t& props, int64_t maxN, int bits, int mode, int
 int heuristicThreads(const hipDeviceProp_t& props, int bits, int mode, int64_t maxN, int64_t chains, int maxP,
                      bool hbm);
 // Completes a group whose bits, mode, hbm, maxP and threads are set: LDS
-// plan, the score-table and register-rich choices, occupancy and grid for
-// nItems chains.  Occupancies come from `ctx`'s cache when it is given, else
-// from the runtime.  Throws Error(InvalidArg / Unsupported).
-void shapeGroup(Context* ctx, const hipDeviceProp_t& props, GroupShape& g, int64_t maxN, int64_t nItems,
+// plan, the score-table, register-rich and rows-in-LDS choices, occupancy and
+// grid for nItems chains.  maxM2: the largest 2m of the group's graphs when
+// every one has compact rows, else -1.  Occupancies come from `ctx`'s cache
+// when it is given, else from the runtime.  Throws Error(InvalidArg /
+// Unsupported).
+void shapeGroup(Context* ctx, const hipDeviceProp_t& props, GroupShape& g, int64_t maxN, int64_t maxM2, int64_t nItems,
                 const LaunchKnobs& knobs);
 
 // What the last launch of a context used (for reports).
@@ -206,6 +215,7 @@ struct LaunchReport {
     int occupancy = 0;
     int mode = 0;   // dev::ScoreMode of the last group
     bool hbm = false;   // the last group's state stayed in HBM
+    bool rows = false;  // the last group read its graph's rows from LDS
     bool tuned = false;
     float trialMsPerSweep[4] = {0, 0, 0, 0};   // by kThreadChoices, 0 if not tried
 };
@@ -294,7 +304,7 @@ public:
     void collect(const ResidentJob& job, const JobSpec& spec, hipStream_t stream);
 
     // Resident workgroups per CU of a kernel variant (cached).
-    int occupancy(int bits, int mode, int threads, size_t lds, bool hbm = false, bool rich = false);
+    int occupancy(int bits, int mode, int threads, size_t lds, bool hbm = false, bool rich = false, bool rows = false);
     // Threads per chain for a launch shape: the autotuner's cached choice,
     // else a heuristic from chains per CU and n.
     int chooseThreads(int bits, int mode, int64_t maxN, int64_t chains, int maxP, bool hbm = false) const;
@@ -327,6 +337,10 @@ private:
     }
     std::unordered_map<uint64_t, int> tuned_;
     LaunchReport last_;
+    // RFMF_PROGRESS set (non-empty, not "0"): runUntil reports each finished
+    // slice on stderr, so hours-long runs show how far they are.
+    bool progress_ = false;
+    void reportProgress(const std::vector<ResidentJob*>& jobs, float kernelMs) const;
 
     int device_;
     hipStream_t stream_ = nullptr;

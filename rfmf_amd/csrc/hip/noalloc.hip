@@ -181,6 +181,9 @@ __global__ __launch_bounds__(256) void buildItems(ItemParams p) {
     it.nRuns = static_cast<int32_t>(end - off);
     it.nPhases = p.nPhases;
     it.deltaMax = p.deltaMax;
+    it.rows16 = nullptr;
+    it.m2 = 0;
+    it.pad = 0;
     p.items[c] = it;
 }
 
@@ -476,9 +479,9 @@ extern "C" rfmf_status rfmf_anneal_tables_noalloc(const rfmf_csr* g, int64_t max
         shape.hbm = !rfmf::fitsLds(props, n, bits, mode, maxP, 128, false);
         shape.threads = rfmf::heuristicThreads(props, bits, mode, n, C, maxP, shape.hbm);
         const rfmf::LaunchKnobs knobs;
-        rfmf::shapeGroup(nullptr, props, shape, n, C, knobs);
+        rfmf::shapeGroup(nullptr, props, shape, n, -1, C, knobs);   // no compact rows in the scratch layout
         rfmf::hipCheck(hipEventRecord(slot.ev0, nullptr), "hipEventRecord");
-        rfmf::hipCheck(rfmf::launch::anneal(bits, shape.mode, shape.threads, shape.hbm, shape.rich, ip.items,
+        rfmf::hipCheck(rfmf::launch::anneal(bits, shape.mode, shape.threads, shape.hbm, shape.rich, false, ip.items,
                                             static_cast<int>(C), reinterpret_cast<uint32_t*>(at(L.sched)),
                                             shape.plan, shape.grid, ~uint64_t{0}, nullptr),
                        "annealSlice launch");

@@ -263,24 +263,30 @@ def test_windows_with_per_phase_autotune_equal_one_shot(graphs):
 def test_score_table_lean_and_rich_builds():
     """The score table's 4-wave tile has a lean build (6 workgroups per CU)
     for launches with more chains than the register-rich build (5 per CU)
-    keeps resident, and the rich build otherwise: a chain's result is the
-    same in both, and the oracle's."""
-    csr = W.torus(40, 50, seed=3)   # n = 2000, 8-bit words
+    keeps resident, and the rich build otherwise; with few chains per CU the
+    rich build also takes the graph's rows into LDS.  A chain's result is the
+    same in every form, and the oracle's."""
+    csr = W.torus(40, 50, seed=3)   # n = 2000, 8-bit words, +/-1 weights
     phases, iters = [128, 8, 1], [40, 40, 40]
     with R.Context(0) as ctx:
         cu = ctx.device_info()["cu"]
         big = make_job(csr, list(range(1, 6 * cu + 1)), phases, iters)
         small = make_job(csr, list(range(1, 49)), phases, iters)
+        rows = make_job(csr, list(range(1, 49)), phases, iters)
         gid = ctx.upload_graph(csr)
         ctx.batch_run([(gid, big)], threads=256, score_table=1)
         geo_big = ctx.last_launch()
-        ctx.batch_run([(gid, small)], threads=256, score_table=1)
+        ctx.batch_run([(gid, small)], threads=256, score_table=1, rows_in_lds=-1)
         geo_small = ctx.last_launch()
-    assert geo_big["score_mode"] == geo_small["score_mode"] == "mul24-table"
+        ctx.batch_run([(gid, rows)], threads=256, score_table=1)
+        geo_rows = ctx.last_launch()
+    assert geo_big["score_mode"] == geo_small["score_mode"] == geo_rows["score_mode"] == "mul24-table"
     assert geo_big["wg_per_cu"] == 6 and geo_small["wg_per_cu"] == 5, (geo_big, geo_small)
-    for a, b in ((big.end_spins, small.end_spins), (big.best_spins, small.best_spins),
-                 (big.best_vals, small.best_vals), (big.final_vals, small.final_vals)):
-        assert np.array_equal(a[:48], b), "lean and rich builds differ"
+    assert not geo_big["rows_in_lds"] and not geo_small["rows_in_lds"] and geo_rows["rows_in_lds"], geo_rows
+    for other in (small, rows):
+        for a, b in ((big.end_spins, other.end_spins), (big.best_spins, other.best_spins),
+                     (big.best_vals, other.best_vals), (big.final_vals, other.final_vals)):
+            assert np.array_equal(a[:48], b), "lean, rich and rows-in-LDS builds differ"
     init = small.init[:8]
     solver = small.seeds[:8]
     beta = R.build_linear_beta_table(0.1, 3.0, sum(iters), 8)

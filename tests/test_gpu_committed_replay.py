@@ -168,16 +168,30 @@ def test_cli_reproduces_the_committed_g62_log():
     args = [str(cli), str(GSET / "G62"), "--seeds", ",".join(str(c["seed"]) for c in chains),
             "--t0s", ",".join(repr(c["t0"]) for c in chains), "--phases", phases, "--iters", iters]
     print("[cli-replay] G62: the committed command, 80 chains x 8e7 sweeps", flush=True)
-    r = subprocess.run(args, capture_output=True, text=True, timeout=1000)
-    assert r.returncode == 0, r.stderr[-2000:]
+    # RFMF_PROGRESS makes the library report every finished launch (one per
+    # phase here, about 40 s apart) on stderr; those lines are echoed as they
+    # come.  stdout (two 7000-character partitions per chain, about 1.1 MB)
+    # goes to a file, so no pipe fills while stderr is read.
+    import tempfile
+    env = dict(os.environ, RFMF_PROGRESS="1")
+    with tempfile.TemporaryFile("w+") as out:
+        p = subprocess.Popen(args, stdout=out, stderr=subprocess.PIPE, text=True, env=env)
+        err = []
+        for line in p.stderr:
+            err.append(line.rstrip("\n"))
+            if line.startswith("rfmf: "):
+                print(f"[cli-replay] {line.rstrip()}", flush=True)
+        rc = p.wait(timeout=60)
+        out.seek(0)
+        stdout = out.read()
+    assert rc == 0, "\n".join(err[-20:])
     mask = lambda s: re.sub(r"time-sec: [0-9.]+", "time-sec: <t>", s)
-    got = [mask(l) for l in r.stdout.splitlines()]
+    got = [mask(l) for l in stdout.splitlines()]
     want = [mask(l) for l in expect]
     bad = [i for i, (a, b) in enumerate(zip(got, want)) if a != b]
     print(f"[cli-replay] G62: {len(got)} stdout lines, {len(want)} expected, {len(bad)} differ "
           f"(first: {bad[:5]})", flush=True)
     assert len(got) == len(want) and not bad
-    err = r.stderr.splitlines()
     for line in rec["log_lines"][:3]:
         assert line in err, line
     for c, ch in enumerate(chains):

@@ -396,3 +396,53 @@ def test_tiny_graphs(n):
     phases, iters = [64, 16, 4, 1], [25, 25, 25, 25]
     beta = R.build_linear_beta_table(0.1, 3.0, sum(iters), 3)
     _case(f"tiny n={csr.n}", csr, [11, 12, 13], beta, phases, iters)
+
+
+# Graphs whose weights are all +1/-1 (n <= 32768) carry compact 2-byte rows;
+# launches with LDS to spare (about one chain per CU) copy them and the
+# offsets into LDS and walk winner rows there.  Forced on and off, every tile
+# (the 2- and 4-wave tiles in their register-rich builds, which carry the
+# path, 8 and 16 waves in every build), against the oracle.
+_ROWS_CASES = {
+    # name: (graph, phases, iters, t0s, expected score mode)
+    "G62-table": ("G62", [128, 64, 8, 1], [50] * 4, [35.0, 50.0], "mul24-table"),
+    "G1-u16": ("G1", [64, 16, 4, 1], [50] * 4, [105.0, 90.0], "int32-mul24"),
+    "G11-int64": ("G11", [8, 4, 2, 1], [50] * 4, [0.005, 0.01], "int64"),
+}
+
+
+@pytest.fixture(scope="module")
+def rows_oracle(gset):
+    cache = {}
+
+    def get(name):
+        if name not in cache:
+            g, phases, iters, t0s, _ = _ROWS_CASES[name]
+            csr = gset(g)
+            seeds = [3, 4]
+            init = np.stack([R.generate_random_spins(s, csr.n) for s in seeds])
+            solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+            beta = _log_tables(t0s, iters)
+            cache[name] = (csr, init, solver, beta, run_oracle_chains(csr, init, solver, beta, phases, iters))
+        return cache[name]
+    return get
+
+
+@pytest.mark.parametrize("name", sorted(_ROWS_CASES))
+@pytest.mark.parametrize("threads", [128, 256, 512, 1024])
+@pytest.mark.parametrize("rows", [-1, 1])
+def test_rows_in_lds_against_oracle(rows_oracle, name, threads, rows):
+    _, phases, iters, _, mode = _ROWS_CASES[name]
+    csr, init, solver, beta, (end, best, cv, bv) = rows_oracle(name)
+    capable = threads >= 512 or mode != "int64"   # int64 scores have no register-rich build
+    print(f"[gpu-parity] rows in LDS: {name} threads={threads} rows_in_lds={rows}", flush=True)
+    job = R.Job(csr, init, solver, beta, phases, iters)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ctx.batch_run([(gid, job)], threads=threads, rows_in_lds=rows)
+        geo = ctx.last_launch()
+    assert geo["threads_per_chain"] == threads and geo["score_mode"] == mode, geo
+    assert geo["rows_in_lds"] == (rows == 1 and capable), geo
+    assert np.array_equal(job.end_spins, end), "final spins differ"
+    assert np.array_equal(job.best_spins, best), "best spins differ"
+    assert np.array_equal(job.final_vals, cv) and np.array_equal(job.best_vals, bv), "cuts differ"

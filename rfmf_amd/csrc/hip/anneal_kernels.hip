@@ -276,6 +276,13 @@ __device__ __forceinline__ uint64_t uni(uint64_t x) {
     return (static_cast<uint64_t>(uni(static_cast<uint32_t>(x >> 32))) << 32) | uni(static_cast<uint32_t>(x));
 }
 __device__ __forceinline__ int64_t uni(int64_t x) { return static_cast<int64_t>(uni(static_cast<uint64_t>(x))); }
+// Zero in a VGPR that the compiler cannot see through: an index offset that
+// makes a wave-uniform load a vector load.
+__device__ __forceinline__ int opaqueZero() {
+    int z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+}
 
 __device__ __forceinline__ uint64_t packKey(int32_t score, int32_t v) {
     if (v < 0) return 0ull;
@@ -524,13 +531,20 @@ constexpr int kSliceWavesPerEu =
 // (graphs whose state does not fit LDS).  FLIP's global atomics complete in
 // L2, so after the sweep's last barrier each wave invalidates its L1 (an
 // agent-scope acquire) before the next sweep reads the words.
+// The rows-in-LDS builds hold one chain per CU (the graph's rows take the
+// LDS), so their floor is the chain's own waves per SIMD: the register budget
+// goes to the schedule (ER n=800 with 64 chains 5% faster than at the
+// register-rich floor of 5).
+template <int NT>
+constexpr int kRowsWavesPerEu = NT >= 512 ? NT / 256 : 1;
+
 // kRows: the build that copies the graph's offsets and compact rows into LDS
 // for items that have them (LdsPlan::rowsOn), for launches of about one chain
 // per CU; only the register-rich 2- and 4-wave tiles and the 8- and 16-wave
 // tiles have it (launch::rowsCapable), and the builds without it are the same
 // code as before the path existed.
 template <int kBits, int kMode, int NT, bool kHbm, bool kRich = false, bool kRows = false>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kSliceWavesPerEu<kBits, kMode, NT, kHbm, kRich>)))
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kRows ? kRowsWavesPerEu<NT> : kSliceWavesPerEu<kBits, kMode, NT, kHbm, kRich>)))
 void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __restrict__ sched, LdsPlan plan,
                  uint64_t sliceSweeps) {
     constexpr int NW = NT / 64;
@@ -722,12 +736,15 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 P = static_cast<int>(uni(phaseP[ph]));
             }
             // The next beta run, fetched now so that its latency hides behind
-            // SELECT (it is consumed after RESOLVE).
-            uint64_t nextEnd = runStop;
+            // SELECT and RESOLVE (it is consumed after RESOLVE).  Vector
+            // loads: a scalar load would share its wait counter with LDS
+            // accesses, so SELECT's first LDS wait would also wait for it.
+            int64_t nextEnd = static_cast<int64_t>(runStop);
             int32_t nextVal = beta;
             if (run < nRuns - 1) {
-                nextEnd = static_cast<uint64_t>(uni(betaEnd[run + 1]));
-                nextVal = uni(betaVal[run + 1]);
+                const int z = opaqueZero();
+                nextEnd = betaEnd[run + 1 + z];
+                nextVal = betaVal[run + 1 + z];
             }
             // The group map without integer division unless P is not a power
             // of two (a mask, else a reciprocal), and the register-argmax
@@ -1068,8 +1085,8 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
             int32_t betaNext = beta;
             if (runStop <= step + 1 && run < nRuns - 1) {
                 ++run;
-                runStop = nextEnd;
-                betaNext = nextVal;
+                runStop = static_cast<uint64_t>(uni(nextEnd));
+                betaNext = uni(nextVal);
                 while (run < nRuns - 1 && runStop <= step + 1) {   // empty runs only
                     ++run;
                     runStop = static_cast<uint64_t>(uni(betaEnd[run]));

@@ -438,6 +438,44 @@ __device__ __forceinline__ void foldLanes(S& s, int32_t& v, int lane) {
     }
 }
 
+// The same reduce-scatter and butterfly steps on packed (score, ~id) keys
+// (packKey): for int32 scores one unsigned 64-bit max is the whole argmax
+// with ties to the smaller id, so a step is the exchange plus one compare and
+// two selects, with no lane-mask logic between them.
+__device__ __forceinline__ void keepMaxKey(uint64_t& a, uint64_t b) { a = b > a ? b : a; }
+template <int kDist>
+__device__ __forceinline__ void scatterStepKey(uint64_t& a, uint64_t b, int lane) {
+    if constexpr (kDist == 8) {
+        const bool upper = (lane & 8) != 0;
+        uint64_t keep = upper ? b : a;
+        const uint64_t send = upper ? a : b;
+        const int64_t s = static_cast<int64_t>(send);
+        keepMaxKey(keep, static_cast<uint64_t>(join64(rowRor<8>(lo32(s)), rowRor<8>(hi32(s)))));
+        a = keep;
+    } else {
+        int32_t xl, yl, xh, yh;
+        swapRows<kDist>(lo32(static_cast<int64_t>(a)), lo32(static_cast<int64_t>(b)), xl, yl);
+        swapRows<kDist>(hi32(static_cast<int64_t>(a)), hi32(static_cast<int64_t>(b)), xh, yh);
+        uint64_t x = static_cast<uint64_t>(join64(xl, xh));
+        keepMaxKey(x, static_cast<uint64_t>(join64(yl, yh)));
+        a = x;
+    }
+}
+template <int kDist>
+__device__ __forceinline__ void foldLanesKey(uint64_t& k, int lane) {
+    const int64_t s = static_cast<int64_t>(k);
+    if constexpr (kDist < 16) {
+        keepMaxKey(k, static_cast<uint64_t>(join64(laneXor<kDist>(lo32(s), lane), laneXor<kDist>(hi32(s), lane))));
+    } else {
+        int32_t la, lb, ha, hb;
+        swapPair<kDist>(lo32(s), la, lb);
+        swapPair<kDist>(hi32(s), ha, hb);
+        uint64_t a = static_cast<uint64_t>(join64(la, ha));
+        keepMaxKey(a, static_cast<uint64_t>(join64(lb, hb)));
+        k = a;
+    }
+}
+
 // Sum over the 64 lanes, left in every lane: rotations inside each 16-lane
 // row, then the row and half swaps (all on the VALU).
 __device__ __forceinline__ int32_t waveSum32(int32_t x) {
@@ -554,6 +592,9 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     constexpr bool kMul24 = kMode == kScore24 || kTab;
     constexpr bool kWide = kMode == kScore64;
     constexpr bool kPackable = !kWide && NT <= 256;   // packed per-lane argmax keys
+    // The per-group argmax folds packed (score, ~id) keys for int32 scores
+    // (fewer dependent steps than (score, id) pairs).
+    constexpr bool kKeyFold = !kWide;
     static_assert(!kRows || (!kHbm && (kRich || NT >= 512)),
                   "rows in LDS: state in LDS, register-rich 2- and 4-wave tiles or 8- and 16-wave tiles");
     static_assert(!kTab || kBits == 8, "the score table needs 8-bit words (field < 128)");
@@ -736,15 +777,22 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 P = static_cast<int>(uni(phaseP[ph]));
             }
             // The next beta run, fetched now so that its latency hides behind
-            // SELECT and RESOLVE (it is consumed after RESOLVE).  Vector
-            // loads: a scalar load would share its wait counter with LDS
-            // accesses, so SELECT's first LDS wait would also wait for it.
+            // SELECT and RESOLVE (it is consumed after RESOLVE).  Vector loads:
+            // a scalar load shares its wait counter with LDS accesses, so
+            // SELECT's first LDS wait would also wait for it.  The lean
+            // 16-bit-word builds (80-VGPR floor) would spill the three VGPRs
+            // and take scalar loads.
             int64_t nextEnd = static_cast<int64_t>(runStop);
             int32_t nextVal = beta;
             if (run < nRuns - 1) {
-                const int z = opaqueZero();
-                nextEnd = betaEnd[run + 1 + z];
-                nextVal = betaVal[run + 1 + z];
+                if constexpr (!(kBits == 16 && !kRich && !kRows)) {
+                    const int z = opaqueZero();
+                    nextEnd = betaEnd[run + 1 + z];
+                    nextVal = betaVal[run + 1 + z];
+                } else {
+                    nextEnd = uni(static_cast<int64_t>(betaEnd[run + 1]));
+                    nextVal = uni(static_cast<int32_t>(betaVal[run + 1]));
+                }
             }
             // The group map without integer division unless P is not a power
             // of two (a mask, else a reciprocal), and the register-argmax
@@ -953,6 +1001,71 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                     // still negative, so it stays empty.
 #pragma unroll
                     for (int j = 0; j < 8; ++j) accV[j] = 8 * accV[j] + j;
+                    if constexpr (kKeyFold) {
+                        // int32 scores: the same reduction on packed keys.
+                        uint64_t kk[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) kk[j] = packKey(accS[j], accV[j]);
+                        if (P >= 8) {
+                            const int Q = P >> 3;
+                            int held = 8, base = 0;
+                            if (Q <= 32) {
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) scatterStepKey<32>(kk[j], kk[j + 4], lane);
+                                held = 4;
+                                base += (lane & 32) ? 4 : 0;
+                            }
+                            if (Q <= 16) {
+#pragma unroll
+                                for (int j = 0; j < 2; ++j) scatterStepKey<16>(kk[j], kk[j + 2], lane);
+                                held = 2;
+                                base += (lane & 16) ? 2 : 0;
+                            }
+                            if (Q <= 8) {
+                                scatterStepKey<8>(kk[0], kk[1], lane);
+                                held = 1;
+                                base += (lane & 8) ? 1 : 0;
+                            }
+                            if (Q <= 4) foldLanesKey<4>(kk[0], lane);
+                            if (Q <= 2) foldLanesKey<2>(kk[0], lane);
+                            if (Q <= 1) foldLanesKey<1>(kk[0], lane);
+                            if ((lane & 7) < Q) {
+                                const int row = wave * pstride + 8 * (lane & (Q - 1)) + base;
+#pragma unroll
+                                for (int k = 0; k < 8; ++k)
+                                    if (k < held) part[row + k] = kk[k];
+                            }
+                        } else {
+                            if (P <= 4) {
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) keepMaxKey(kk[j], kk[j + 4]);
+                            }
+                            if (P <= 2) {
+#pragma unroll
+                                for (int j = 0; j < 2; ++j) keepMaxKey(kk[j], kk[j + 2]);
+                            }
+                            if (P == 1) keepMaxKey(kk[0], kk[1]);
+                            int base = 0;
+                            if (P == 4) {
+                                scatterStepKey<32>(kk[0], kk[2], lane);
+                                scatterStepKey<32>(kk[1], kk[3], lane);
+                                base += (lane & 32) ? 2 : 0;
+                                scatterStepKey<16>(kk[0], kk[1], lane);
+                                base += (lane & 16) ? 1 : 0;
+                            } else if (P == 2) {
+                                scatterStepKey<32>(kk[0], kk[1], lane);
+                                base += (lane & 32) ? 1 : 0;
+                            }
+                            if (P == 1) foldLanesKey<32>(kk[0], lane);
+                            if (P <= 2) foldLanesKey<16>(kk[0], lane);
+                            foldLanesKey<8>(kk[0], lane);
+                            foldLanesKey<4>(kk[0], lane);
+                            foldLanesKey<2>(kk[0], lane);
+                            foldLanesKey<1>(kk[0], lane);
+                            if ((lane & (63 >> (P >> 1))) == 0) part[wave * pstride + base] = kk[0];
+                        }
+                        return;
+                    }
                     if (P >= 8) {
                         // Lanes l and l' share groups iff l = l' (mod Q); slot j
                         // of lane l feeds group 8 (l mod Q) + j.  Reduce-scatter

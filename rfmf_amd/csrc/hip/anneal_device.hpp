@@ -60,7 +60,7 @@ struct alignas(16) ChainItem {
     int32_t deltaMax;          // bound on |delta| (max weighted degree, capped at 2^31 - 1)
     const uint16_t* rows16;    // compact rows (nullptr when the graph has none)
     uint32_t m2;               // directed edges (2m)
-    int32_t pad;
+    int32_t maxDeg;            // longest row (0: not known)
 };
 static_assert(sizeof(ChainItem) == 128, "ChainItem layout");
 

@@ -756,6 +756,15 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
         const int lanesBits = lanesK <= 1 ? 0 : 32 - __clz(lanesK - 1);
         const int lanesFull = nFull / NT;   // full blocks every lane scores
         const uint64_t deltaMax = static_cast<uint64_t>(uni(it.deltaMax));
+        // log2 of the lanes that walk the longest row two edges at a time
+        // (ceil(maxDeg / 2) rounded up to a power of two; 64 when unknown).
+        const int rowLanesLg = [&]() {
+            const int d = uni(it.maxDeg);
+            if (d <= 0) return 6;
+            const int half = (d + 1) >> 1;
+            const int lg = half <= 1 ? 0 : 32 - __clz(half - 1);
+            return lg > 6 ? 6 : lg;
+        }();
         // Best-spin snapshot: the spins after an improving sweep, taken before
         // the next sweep's flips (and at the end of the slice).  Each thread
         // packs its own blocks; nothing else touches bl8 until the copy-out.
@@ -848,8 +857,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
             // best score and, among equal scores, the earlier block (the
             // smaller vertex).  Exact while |score| < 2^(31 - packShift),
             // which the per-sweep bound guarantees.
-            auto scorePackedLanes = [&](Score (&accS)[8], int32_t (&accV)[8]) {
-                int32_t acc[8];
+            auto scorePackedLanes = [&](int32_t (&acc)[8]) {
 #pragma unroll
                 for (int j = 0; j < 8; ++j) acc[j] = INT32_MIN;
                 const int cb = kPackable ? packShift : 0;
@@ -930,18 +938,12 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 asm volatile("" : "+v"(c));
                 int b = tid;
                 for (int k = 0; k < lanesFull; ++k, b += NT, --c) scoreBlockPacked(b, c, false);
-                if (b < nFull) scoreBlockPacked(b, c, false);
-                if (nFull < nBlk && tid == (nFull % NT)) scoreBlockPacked(nFull, c, true);
-                // Every lane scored at least one full block, so no slot is
-                // empty.  Decode: score = key >> shift (plus bias * beta for
-                // the table), block = tid + (K - 1 - low bits) * NT.
-                const uint32_t mask = (1u << cb) - 1u;
-                const int32_t bTop = tid + (lanesK - 1) * NT;
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    accS[j] = static_cast<Score>(static_cast<uint32_t>(acc[j] >> cb) + biasBeta);
-                    accV[j] = bTop - static_cast<int32_t>(static_cast<uint32_t>(acc[j]) & mask) * NT;
-                }
+                // The lanes below nFull mod NT score one more full block, and
+                // lane nFull mod NT the partial one (block nFull, the same
+                // position k = lanesFull).
+                const int extra = nFull - lanesFull * NT;
+                if (tid < extra) scoreBlockPacked(tid + lanesFull * NT, c, false);
+                if (nFull < nBlk && tid == extra) scoreBlockPacked(nFull, c, true);
             };
             auto select = [&](auto laneTag) {
                 constexpr bool kLane = decltype(laneTag)::value;
@@ -950,11 +952,88 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 int32_t accV[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) { accS[j] = kNoScore; accV[j] = -1; }
-                bool scored = false;
+                // int32 scores: the per-group argmax folds packed 64-bit keys
+                // across lanes and publishes one per group and wave.
+                auto foldKeys = [&](uint64_t (&kk)[8]) {
+                    if (P >= 8) {
+                        const int Q = P >> 3;
+                        int held = 8, base = 0;
+                        if (Q <= 32) {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) scatterStepKey<32>(kk[j], kk[j + 4], lane);
+                            held = 4;
+                            base += (lane & 32) ? 4 : 0;
+                        }
+                        if (Q <= 16) {
+#pragma unroll
+                            for (int j = 0; j < 2; ++j) scatterStepKey<16>(kk[j], kk[j + 2], lane);
+                            held = 2;
+                            base += (lane & 16) ? 2 : 0;
+                        }
+                        if (Q <= 8) {
+                            scatterStepKey<8>(kk[0], kk[1], lane);
+                            held = 1;
+                            base += (lane & 8) ? 1 : 0;
+                        }
+                        if (Q <= 4) foldLanesKey<4>(kk[0], lane);
+                        if (Q <= 2) foldLanesKey<2>(kk[0], lane);
+                        if (Q <= 1) foldLanesKey<1>(kk[0], lane);
+                        if ((lane & 7) < Q) {
+                            const int row = wave * pstride + 8 * (lane & (Q - 1)) + base;
+#pragma unroll
+                            for (int k = 0; k < 8; ++k)
+                                if (k < held) part[row + k] = kk[k];
+                        }
+                    } else {
+                        if (P <= 4) {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) keepMaxKey(kk[j], kk[j + 4]);
+                        }
+                        if (P <= 2) {
+#pragma unroll
+                            for (int j = 0; j < 2; ++j) keepMaxKey(kk[j], kk[j + 2]);
+                        }
+                        if (P == 1) keepMaxKey(kk[0], kk[1]);
+                        int base = 0;
+                        if (P == 4) {
+                            scatterStepKey<32>(kk[0], kk[2], lane);
+                            scatterStepKey<32>(kk[1], kk[3], lane);
+                            base += (lane & 32) ? 2 : 0;
+                            scatterStepKey<16>(kk[0], kk[1], lane);
+                            base += (lane & 16) ? 1 : 0;
+                        } else if (P == 2) {
+                            scatterStepKey<32>(kk[0], kk[1], lane);
+                            base += (lane & 32) ? 1 : 0;
+                        }
+                        if (P == 1) foldLanesKey<32>(kk[0], lane);
+                        if (P <= 2) foldLanesKey<16>(kk[0], lane);
+                        foldLanesKey<8>(kk[0], lane);
+                        foldLanesKey<4>(kk[0], lane);
+                        foldLanesKey<2>(kk[0], lane);
+                        foldLanesKey<1>(kk[0], lane);
+                        if ((lane & (63 >> (P >> 1))) == 0) part[wave * pstride + base] = kk[0];
+                    }
+                };
                 if constexpr (kLane && kPackable) {
                     if (packShift >= 0) {
-                        scorePackedLanes(accS, accV);
-                        scored = true;
+                        // The lane keys fold as they are: the high word is the
+                        // key (score, then the earlier block k), biased to
+                        // unsigned, and the low word ~(8 tid + j) breaks the
+                        // remaining ties toward the smaller lane and slot.
+                        // (score, k, tid, j) orders as (score, vertex) does,
+                        // since vertex = 8 (tid + k NT) + j; RESOLVE recovers
+                        // k from the key's low packShift bits.  Every lane has
+                        // full blocks, so every wave is busy and no slot is
+                        // empty.
+                        int32_t acc[8];
+                        scorePackedLanes(acc);
+                        uint64_t kk[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j)
+                            kk[j] = (static_cast<uint64_t>(static_cast<uint32_t>(acc[j]) ^ 0x80000000u) << 32) |
+                                    static_cast<uint64_t>(~static_cast<uint32_t>(8 * tid + j));
+                        foldKeys(kk);
+                        return;
                     }
                 }
                 auto scoreBlock = [&](int b, bool tail) {
@@ -990,10 +1069,8 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                         }
                     }
                 };
-                if (!scored) {
-                    for (int b = tid; b < nFull; b += NT) scoreBlock(b, false);
-                    if (nFull < nBlk && tid == (nFull % NT)) scoreBlock(nFull, true);
-                }
+                for (int b = tid; b < nFull; b += NT) scoreBlock(b, false);
+                if (nFull < nBlk && tid == (nFull % NT)) scoreBlock(nFull, true);
 
                 if constexpr (kLane) {
                     if (wave >= busyWaves) return;
@@ -1006,64 +1083,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                         uint64_t kk[8];
 #pragma unroll
                         for (int j = 0; j < 8; ++j) kk[j] = packKey(accS[j], accV[j]);
-                        if (P >= 8) {
-                            const int Q = P >> 3;
-                            int held = 8, base = 0;
-                            if (Q <= 32) {
-#pragma unroll
-                                for (int j = 0; j < 4; ++j) scatterStepKey<32>(kk[j], kk[j + 4], lane);
-                                held = 4;
-                                base += (lane & 32) ? 4 : 0;
-                            }
-                            if (Q <= 16) {
-#pragma unroll
-                                for (int j = 0; j < 2; ++j) scatterStepKey<16>(kk[j], kk[j + 2], lane);
-                                held = 2;
-                                base += (lane & 16) ? 2 : 0;
-                            }
-                            if (Q <= 8) {
-                                scatterStepKey<8>(kk[0], kk[1], lane);
-                                held = 1;
-                                base += (lane & 8) ? 1 : 0;
-                            }
-                            if (Q <= 4) foldLanesKey<4>(kk[0], lane);
-                            if (Q <= 2) foldLanesKey<2>(kk[0], lane);
-                            if (Q <= 1) foldLanesKey<1>(kk[0], lane);
-                            if ((lane & 7) < Q) {
-                                const int row = wave * pstride + 8 * (lane & (Q - 1)) + base;
-#pragma unroll
-                                for (int k = 0; k < 8; ++k)
-                                    if (k < held) part[row + k] = kk[k];
-                            }
-                        } else {
-                            if (P <= 4) {
-#pragma unroll
-                                for (int j = 0; j < 4; ++j) keepMaxKey(kk[j], kk[j + 4]);
-                            }
-                            if (P <= 2) {
-#pragma unroll
-                                for (int j = 0; j < 2; ++j) keepMaxKey(kk[j], kk[j + 2]);
-                            }
-                            if (P == 1) keepMaxKey(kk[0], kk[1]);
-                            int base = 0;
-                            if (P == 4) {
-                                scatterStepKey<32>(kk[0], kk[2], lane);
-                                scatterStepKey<32>(kk[1], kk[3], lane);
-                                base += (lane & 32) ? 2 : 0;
-                                scatterStepKey<16>(kk[0], kk[1], lane);
-                                base += (lane & 16) ? 1 : 0;
-                            } else if (P == 2) {
-                                scatterStepKey<32>(kk[0], kk[1], lane);
-                                base += (lane & 32) ? 1 : 0;
-                            }
-                            if (P == 1) foldLanesKey<32>(kk[0], lane);
-                            if (P <= 2) foldLanesKey<16>(kk[0], lane);
-                            foldLanesKey<8>(kk[0], lane);
-                            foldLanesKey<4>(kk[0], lane);
-                            foldLanesKey<2>(kk[0], lane);
-                            foldLanesKey<1>(kk[0], lane);
-                            if ((lane & (63 >> (P >> 1))) == 0) part[wave * pstride + base] = kk[0];
-                        }
+                        foldKeys(kk);
                         return;
                     }
                     if (P >= 8) {
@@ -1151,6 +1171,8 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 }
             }
             __syncthreads();   // B1: all partials / slots written
+            const bool packedKeys = kPackable && laneReduce && packShift >= 0;
+            const uint32_t packMask = packShift > 0 ? (1u << packShift) - 1u : 0u;
 
             // ---------------- RESOLVE: winner of each group ----------------
             for (int g = tid; g < P; g += NT) {
@@ -1184,6 +1206,11 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 Delta dl = 0;
                 if (key != 0ull) {
                     pick = static_cast<int32_t>(~(kWide ? kid : static_cast<uint32_t>(key)));
+                    // Lane keys carry 8 tid + j; the block position k is
+                    // K - 1 minus the key's low packShift bits.
+                    if (packedKeys)
+                        pick += (lanesK - 1 - static_cast<int32_t>(static_cast<uint32_t>(key >> 32) & packMask))
+                                << (3 + kLgNT);
                     const Word w = ldsWord<kBits, kHbm>(wl, pick);
                     sp = (w & F::kSpin) ? 1 : -1;
                     dl = static_cast<Delta>(w & F::kField) - static_cast<Delta>(F::kBias);
@@ -1214,7 +1241,11 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 // 2^lgL = 2^floor(log2(threads / P)) lanes per winner (at most
                 // a wave), from ceil(log2 P) without a division.
                 const int ceilLgP = P <= 1 ? 0 : 32 - __clz(P - 1);
-                const int lgL = kLgNT - ceilLgP < 0 ? 0 : (kLgNT - ceilLgP > 6 ? 6 : kLgNT - ceilLgP);
+                int lgL = kLgNT - ceilLgP < 0 ? 0 : (kLgNT - ceilLgP > 6 ? 6 : kLgNT - ceilLgP);
+                // No more lanes than the longest row has edge pairs: they
+                // would have nothing to walk, and the winners then fit in
+                // fewer waves (the rest leave FLIP at once).
+                lgL = lgL < rowLanesLg ? lgL : rowLanesLg;
                 const int perPass = NT >> lgL;
                 const int passes = (P + perPass - 1) >> (kLgNT - lgL);
                 const uint32_t pRecip = recipOfP();

@@ -154,9 +154,10 @@ HostGraph prepareGraph(const uint32_t* offsets, const int32_t* to, const int32_t
     HostGraph g;
     g.n = n;
     g.m2 = m2;
-    int64_t B = 0;
+    int64_t B = 0, maxDeg = 0;
     for (int64_t v = 0; v < n; ++v) {
         if (offsets[v + 1] < offsets[v]) throw Error(Status::InvalidArg, "uploadGraph: offsets not monotone");
+        maxDeg = std::max<int64_t>(maxDeg, offsets[v + 1] - offsets[v]);
         int64_t s = 0;
         for (uint32_t i = offsets[v]; i < offsets[v + 1]; ++i) {
             if (to[i] < 0 || to[i] >= n) throw Error(Status::InvalidArg, "uploadGraph: neighbour id out of range");
@@ -165,6 +166,7 @@ HostGraph prepareGraph(const uint32_t* offsets, const int32_t* to, const int32_t
         B = std::max(B, s);
     }
     g.B = B;
+    g.maxDeg = maxDeg;
     g.offsets.assign(offsets, offsets + n + 1);
     g.to.assign(to, to + m2);
     g.w.assign(w, w + m2);
@@ -197,6 +199,7 @@ int32_t Context::uploadGraph(HostGraph&& hg, hipStream_t async) {
     g->n = hg.n;
     g->m2 = hg.m2;
     g->B = hg.B;
+    g->maxDeg = hg.maxDeg;
     g->host = std::move(hg);
     const int64_t n = g->n, m2 = g->m2;
     hipCheck(hipSetDevice(device_), "hipSetDevice");
@@ -504,6 +507,7 @@ std::unique_ptr<ResidentJob> Context::placeJob(const HostJob& hj, const DevGraph
         it.deltaMax = static_cast<int32_t>(std::min<int64_t>(g->B, INT32_MAX));
         it.rows16 = g->rows16.bytes() ? g->rows16.as<uint16_t>() : nullptr;
         it.m2 = static_cast<uint32_t>(g->m2);
+        it.maxDeg = static_cast<int32_t>(std::min<int64_t>(g->maxDeg, INT32_MAX));
     }
     return job;
 }

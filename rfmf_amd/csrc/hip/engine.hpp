@@ -78,6 +78,7 @@ struct HostGraph {
     int64_t n = 0;
     int64_t m2 = 0;
     int64_t B = 0;
+    int64_t maxDeg = 0;   // longest row (unweighted degree)
     std::vector<uint32_t> offsets;
     std::vector<int32_t> to, w;
     std::vector<int32_t> edges;   // int2[m2]
@@ -97,6 +98,7 @@ struct DevGraph {
     int64_t n = 0;
     int64_t m2 = 0;
     int64_t B = 0;    // max weighted degree
+    int64_t maxDeg = 0;   // longest row
     HostGraph host;   // host copy for the initial cut
 };
 

@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void buildItems(ItemParams p) {
     it.deltaMax = p.deltaMax;
     it.rows16 = nullptr;
     it.m2 = 0;
-    it.pad = 0;
+    it.maxDeg = 0;
     p.items[c] = it;
 }
 

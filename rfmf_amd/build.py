@@ -29,8 +29,11 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-paramet
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 HOST_SOURCES = ["host/graph.cpp", "host/read_gset.cpp", "host/max_cut.cpp", "host/host_common.cpp"]
-HIP_SOURCES = ["hip/anneal_kernels.hip", "hip/engine.cpp", "hip/capi.cpp", "hip/server.cpp", "hip/tune_flips.cpp",
-               "hip/noalloc.hip"]
+# The slice kernel's variants compile in one unit per word width and score
+# mode (slice_w*.hip), in parallel.
+SLICE_SOURCES = sorted(f"hip/{p.name}" for p in (CSRC / "hip").glob("slice_w*.hip"))
+HIP_SOURCES = ["hip/anneal_kernels.hip", *SLICE_SOURCES, "hip/engine.cpp", "hip/capi.cpp", "hip/server.cpp",
+               "hip/tune_flips.cpp", "hip/noalloc.hip"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -59,7 +62,7 @@ def build(jobs: int | None = None) -> Path:
     CLI.parent.mkdir(parents=True, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, HOST_SOURCES + HIP_SOURCES))
+        objs = list(ex.map(_compile, HIP_SOURCES + HOST_SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, *HIP_FLAGS, "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread"])
     os.replace(tmp, LIB)

@@ -42,9 +42,8 @@ struct LdsPlan {
     uint32_t bestOff;     // best-spin bitmask
     uint32_t partOff;     // per-wave group partials / atomic slots (u64)
     uint32_t partIdOff;   // int64 scores only: ~id beside each partial (u32)
-    uint32_t pickOff;     // int32 winner id per group
-    uint32_t spinOff;     // int32 winner pre-flip spin per group
-    uint32_t deltaOff;    // int32 winner pre-flip delta per group
+    uint32_t pickOff;     // {winner id, pre-flip spin} per group (int32 pairs)
+    uint32_t deltaOff;    // winner pre-flip delta per group (int32, int64 for 64-bit words)
     uint32_t dpartOff;    // int64 per-wave cut-delta partials
     uint32_t miscOff;     // scheduler broadcast
     uint32_t rowsOff;     // graph offsets u32[n+1] (rowsOn only)
@@ -86,8 +85,7 @@ inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads, int mo
     p.partOff = off;  off += alignUp(static_cast<uint32_t>(slots) * 8u, 16);
     p.partIdOff = off;
     if (mode == kScore64) off += alignUp(static_cast<uint32_t>(slots) * 4u, 16);
-    p.pickOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
-    p.spinOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 4u, 16);
+    p.pickOff = off;  off += alignUp(static_cast<uint32_t>(maxP) * 8u, 16);
     p.deltaOff = off; off += alignUp(static_cast<uint32_t>(maxP) * (bits == 64 ? 8u : 4u), 16);
     p.dpartOff = off; off += alignUp(static_cast<uint32_t>(waves) * 8u, 16);
     p.miscOff = off;  off += 16;
@@ -105,6 +103,13 @@ inline LdsPlan makeLdsPlan(int64_t maxN, int bits, int maxP, int threads, int mo
 }  // namespace dev
 
 namespace launch {
+
+// Host stub of the slice kernel variant for `threads` threads per chain (and
+// the register-rich and rows-in-LDS builds), or nullptr when that variant is
+// not built.  Defined in anneal_slice.hpp and instantiated, one word width
+// and score mode per unit, in the slice_w*.hip files.
+template <int kBits, int kMode, bool kHbm>
+const void* sliceKernel(int threads, bool rich, bool rows);
 
 // Persistent slice kernel over `nItems` chain items, each advanced by at most
 // `sliceSweeps` sweeps, `threads` threads per workgroup; `sched` is a 2-word

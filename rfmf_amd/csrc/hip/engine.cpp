@@ -582,7 +582,7 @@ void shapeGroup(Context* ctx, const hipDeviceProp_t& props, GroupShape& g, int64
         }
     }
     // The 24-bit score variants' 2- and 4-wave tiles have a register-rich
-    // build (5 workgroups per CU instead of 6): taken when it keeps every
+    // build (4 workgroups per CU instead of 6): taken when it keeps every
     // chain resident that the lean one would.
     g.rich = false;
     if ((g.mode == dev::kScoreTab || g.mode == dev::kScore24) && !g.hbm && g.threads <= 256) {

@@ -186,7 +186,7 @@ struct GroupShape {
     int occupancy = 1;   // resident workgroups per CU
     int maxP = 1;
     bool hbm = false;
-    bool rich = false;   // 24-bit-score build for 5 workgroups per CU (launch::anneal)
+    bool rich = false;   // 24-bit-score build for 4 workgroups per CU (launch::anneal)
     bool rows = false;   // plan.rowsOn: graphs with compact rows read them from LDS
 };
 

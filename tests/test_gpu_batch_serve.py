@@ -262,7 +262,7 @@ def test_windows_with_per_phase_autotune_equal_one_shot(graphs):
 
 def test_score_table_lean_and_rich_builds():
     """The score table's 4-wave tile has a lean build (6 workgroups per CU)
-    for launches with more chains than the register-rich build (5 per CU)
+    for launches with more chains than the register-rich build (4 per CU)
     keeps resident, and the rich build otherwise; with few chains per CU the
     rich build also takes the graph's rows into LDS.  A chain's result is the
     same in every form, and the oracle's."""
@@ -281,7 +281,7 @@ def test_score_table_lean_and_rich_builds():
         ctx.batch_run([(gid, rows)], threads=256, score_table=1)
         geo_rows = ctx.last_launch()
     assert geo_big["score_mode"] == geo_small["score_mode"] == geo_rows["score_mode"] == "mul24-table"
-    assert geo_big["wg_per_cu"] == 6 and geo_small["wg_per_cu"] == 5, (geo_big, geo_small)
+    assert geo_big["wg_per_cu"] == 6 and geo_small["wg_per_cu"] == 4, (geo_big, geo_small)
     assert not geo_big["rows_in_lds"] and not geo_small["rows_in_lds"] and geo_rows["rows_in_lds"], geo_rows
     for other in (small, rows):
         for a, b in ((big.end_spins, other.end_spins), (big.best_spins, other.best_spins),

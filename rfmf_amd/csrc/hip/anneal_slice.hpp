@@ -667,17 +667,19 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     if constexpr (kTab) {
         for (int i = threadIdx.x; i < 2048; i += NT) gtab[i] = kGumbelCoarse[i & 15] + ((i >> 4) & 0x7F);
     }
+    // Lane t < 16 keeps LUT entry t in a register, so the per-sweep rebuild
+    // reads no memory.
+    const int32_t lutCoarse = !kTab && threadIdx.x < 16 ? kGumbelCoarse[threadIdx.x] : 0;
     auto setLut = [&](int32_t b) {
         if constexpr (kTab) return;
         const int t = threadIdx.x;
         if (t < 16) {
             if constexpr (kWide)
-                lutw[t] = static_cast<int64_t>(static_cast<uint64_t>(static_cast<int64_t>(kGumbelCoarse[t])) -
+                lutw[t] = static_cast<int64_t>(static_cast<uint64_t>(static_cast<int64_t>(lutCoarse)) -
                                                static_cast<uint64_t>(F::kBias) *
                                                    static_cast<uint64_t>(static_cast<int64_t>(b)));
             else
-                lutb[t] = static_cast<uint32_t>(kGumbelCoarse[t]) -
-                          static_cast<uint32_t>(F::kBias) * static_cast<uint32_t>(b);
+                lutb[t] = static_cast<uint32_t>(lutCoarse) - static_cast<uint32_t>(F::kBias) * static_cast<uint32_t>(b);
         }
     };
     int64_t* dpart = reinterpret_cast<int64_t*>(lds + plan.dpartOff);
@@ -989,65 +991,84 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 // int32 scores: the per-group argmax folds packed 64-bit keys
                 // across lanes and publishes one per group and wave.
                 auto foldKeys = [&](uint64_t (&kk)[8]) {
-                    if (P >= 8) {
-                        const int Q = P >> 3;
-                        int held = 8, base = 0;
-                        if (Q <= 32) {
+                    // One straight-line reduction per group count (P a power
+                    // of two up to 512): no branches between the steps.
+                    auto foldP = [&](auto lgTag) {
+                        constexpr int kP = 1 << decltype(lgTag)::value;
+                        if constexpr (kP >= 8) {
+                            constexpr int Q = kP >> 3;
+                            constexpr int held = Q <= 8 ? 1 : Q <= 16 ? 2 : Q <= 32 ? 4 : 8;
+                            int base = 0;
+                            if constexpr (Q <= 32) {
 #pragma unroll
-                            for (int j = 0; j < 4; ++j) scatterStepKey<32>(kk[j], kk[j + 4], lane);
-                            held = 4;
-                            base += (lane & 32) ? 4 : 0;
-                        }
-                        if (Q <= 16) {
+                                for (int j = 0; j < 4; ++j) scatterStepKey<32>(kk[j], kk[j + 4], lane);
+                                base += (lane & 32) ? 4 : 0;
+                            }
+                            if constexpr (Q <= 16) {
 #pragma unroll
-                            for (int j = 0; j < 2; ++j) scatterStepKey<16>(kk[j], kk[j + 2], lane);
-                            held = 2;
-                            base += (lane & 16) ? 2 : 0;
-                        }
-                        if (Q <= 8) {
-                            scatterStepKey<8>(kk[0], kk[1], lane);
-                            held = 1;
-                            base += (lane & 8) ? 1 : 0;
-                        }
-                        if (Q <= 4) foldLanesKey<4>(kk[0], lane);
-                        if (Q <= 2) foldLanesKey<2>(kk[0], lane);
-                        if (Q <= 1) foldLanesKey<1>(kk[0], lane);
-                        if ((lane & 7) < Q) {
-                            const int row = wave * pstride + 8 * (lane & (Q - 1)) + base;
+                                for (int j = 0; j < 2; ++j) scatterStepKey<16>(kk[j], kk[j + 2], lane);
+                                base += (lane & 16) ? 2 : 0;
+                            }
+                            if constexpr (Q <= 8) {
+                                scatterStepKey<8>(kk[0], kk[1], lane);
+                                base += (lane & 8) ? 1 : 0;
+                            }
+                            if constexpr (Q <= 4) foldLanesKey<4>(kk[0], lane);
+                            if constexpr (Q <= 2) foldLanesKey<2>(kk[0], lane);
+                            if constexpr (Q <= 1) foldLanesKey<1>(kk[0], lane);
+                            if ((lane & 7) < Q) {
+                                const int row = wave * pstride + 8 * (lane & (Q - 1)) + base;
 #pragma unroll
-                            for (int k = 0; k < 8; ++k)
-                                if (k < held) part[row + k] = kk[k];
-                        }
-                    } else {
-                        if (P <= 4) {
+                                for (int k = 0; k < held; ++k) part[row + k] = kk[k];
+                            }
+                        } else {
+                            if constexpr (kP <= 4) {
 #pragma unroll
-                            for (int j = 0; j < 4; ++j) keepMaxKey(kk[j], kk[j + 4]);
-                        }
-                        if (P <= 2) {
+                                for (int j = 0; j < 4; ++j) keepMaxKey(kk[j], kk[j + 4]);
+                            }
+                            if constexpr (kP <= 2) {
 #pragma unroll
-                            for (int j = 0; j < 2; ++j) keepMaxKey(kk[j], kk[j + 2]);
+                                for (int j = 0; j < 2; ++j) keepMaxKey(kk[j], kk[j + 2]);
+                            }
+                            if constexpr (kP == 1) keepMaxKey(kk[0], kk[1]);
+                            int base = 0;
+                            if constexpr (kP == 4) {
+                                scatterStepKey<32>(kk[0], kk[2], lane);
+                                scatterStepKey<32>(kk[1], kk[3], lane);
+                                base += (lane & 32) ? 2 : 0;
+                                scatterStepKey<16>(kk[0], kk[1], lane);
+                                base += (lane & 16) ? 1 : 0;
+                            } else if constexpr (kP == 2) {
+                                scatterStepKey<32>(kk[0], kk[1], lane);
+                                base += (lane & 32) ? 1 : 0;
+                            }
+                            if constexpr (kP == 1) foldLanesKey<32>(kk[0], lane);
+                            if constexpr (kP <= 2) foldLanesKey<16>(kk[0], lane);
+                            foldLanesKey<8>(kk[0], lane);
+                            foldLanesKey<4>(kk[0], lane);
+                            foldLanesKey<2>(kk[0], lane);
+                            foldLanesKey<1>(kk[0], lane);
+                            if ((lane & (63 >> (kP >> 1))) == 0) part[wave * pstride + base] = kk[0];
                         }
-                        if (P == 1) keepMaxKey(kk[0], kk[1]);
-                        int base = 0;
-                        if (P == 4) {
-                            scatterStepKey<32>(kk[0], kk[2], lane);
-                            scatterStepKey<32>(kk[1], kk[3], lane);
-                            base += (lane & 32) ? 2 : 0;
-                            scatterStepKey<16>(kk[0], kk[1], lane);
-                            base += (lane & 16) ? 1 : 0;
-                        } else if (P == 2) {
-                            scatterStepKey<32>(kk[0], kk[1], lane);
-                            base += (lane & 32) ? 1 : 0;
-                        }
-                        if (P == 1) foldLanesKey<32>(kk[0], lane);
-                        if (P <= 2) foldLanesKey<16>(kk[0], lane);
-                        foldLanesKey<8>(kk[0], lane);
-                        foldLanesKey<4>(kk[0], lane);
-                        foldLanesKey<2>(kk[0], lane);
-                        foldLanesKey<1>(kk[0], lane);
-                        if ((lane & (63 >> (P >> 1))) == 0) part[wave * pstride + base] = kk[0];
+                    };
+                    switch (P) {
+                        case 1: foldP(std::integral_constant<int, 0>{}); break;
+                        case 2: foldP(std::integral_constant<int, 1>{}); break;
+                        case 4: foldP(std::integral_constant<int, 2>{}); break;
+                        case 8: foldP(std::integral_constant<int, 3>{}); break;
+                        case 16: foldP(std::integral_constant<int, 4>{}); break;
+                        case 32: foldP(std::integral_constant<int, 5>{}); break;
+                        case 64: foldP(std::integral_constant<int, 6>{}); break;
+                        case 128: foldP(std::integral_constant<int, 7>{}); break;
+                        case 256: foldP(std::integral_constant<int, 8>{}); break;
+                        default: foldP(std::integral_constant<int, 9>{}); break;   // 512
                     }
                 };
+                // int32 scores on the lane path: the lane keys (packed, or
+                // (score, block) pairs when the packed bound fails) become
+                // 64-bit keys and take the one reduction above.
+                uint64_t kk[8];
+                bool haveKeys = false;
                 if constexpr (kLane && kPackable) {
                     if (packShift >= 0) {
                         // The lane keys fold as they are: the high word is the
@@ -1062,17 +1083,14 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                         // lane has no full block, and becomes the empty key 0.
                         int32_t acc[8];
                         scorePackedLanes(acc);
-                        if (wave >= busyWaves) return;
                         const bool mayBeEmpty = lanesFull == 0;
-                        uint64_t kk[8];
 #pragma unroll
                         for (int j = 0; j < 8; ++j) {
                             kk[j] = (static_cast<uint64_t>(static_cast<uint32_t>(acc[j]) ^ 0x80000000u) << 32) |
                                     static_cast<uint64_t>(~static_cast<uint32_t>(8 * tid + j));
                             if (mayBeEmpty && acc[j] == INT32_MIN) kk[j] = 0ull;
                         }
-                        foldKeys(kk);
-                        return;
+                        haveKeys = true;
                     }
                 }
                 auto scoreBlock = [&](int b, bool tail) {
@@ -1108,20 +1126,24 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                         }
                     }
                 };
-                for (int b = tid; b < nFull; b += NT) scoreBlock(b, false);
-                if (nFull < nBlk && tid == (nFull % NT)) scoreBlock(nFull, true);
+                if (!haveKeys) {
+                    for (int b = tid; b < nFull; b += NT) scoreBlock(b, false);
+                    if (nFull < nBlk && tid == (nFull % NT)) scoreBlock(nFull, true);
+                    if constexpr (kLane) {
+                        // Block index -> vertex; an empty slot's -1 becomes
+                        // j - 8, still negative, so it stays empty.
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) accV[j] = 8 * accV[j] + j;
+                        if constexpr (kKeyFold) {
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) kk[j] = packKey(accS[j], accV[j]);
+                        }
+                    }
+                }
 
                 if constexpr (kLane) {
                     if (wave >= busyWaves) return;
-                    // Block index -> vertex; an empty slot's -1 becomes j - 8,
-                    // still negative, so it stays empty.
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) accV[j] = 8 * accV[j] + j;
                     if constexpr (kKeyFold) {
-                        // int32 scores: the same reduction on packed keys.
-                        uint64_t kk[8];
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) kk[j] = packKey(accS[j], accV[j]);
                         foldKeys(kk);
                         return;
                     }
@@ -1336,10 +1358,13 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                                 const i32x2 r1 = pks[grp(e1.x)];
                                 const Word wd0 = ldsWord<kBits, kHbm>(wl, e0.x);
                                 const Word wd1 = ldsWord<kBits, kHbm>(wl, e1.x);
+                                // Spins are +/-1: the products 2 w s_b and
+                                // its coupling w2 s_x are sign selections.
                                 auto apply = [&](const i32x2& e, const i32x2& r, Word wd, bool live) {
-                                    const Delta w2 = live ? Delta{2} * e.y * sb : Delta{0};
+                                    const Delta w = live ? static_cast<Delta>(e.y) : Delta{0};
+                                    const Delta w2 = sb > 0 ? w + w : -(w + w);
                                     const bool isW = r.x == e.x;
-                                    const Delta t = w2 * r.y;
+                                    const Delta t = r.y > 0 ? w2 : -w2;
                                     if (isW && grp(e.x) < g) dsum -= t;
                                     Delta add = (wd & F::kSpin) ? -w2 : w2;
                                     int at = e.x;

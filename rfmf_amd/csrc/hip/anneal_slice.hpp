@@ -628,10 +628,17 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     constexpr bool kTab = kMode == kScoreTab;
     constexpr bool kMul24 = kMode == kScore24 || kTab;
     constexpr bool kWide = kMode == kScore64;
-    constexpr bool kPackable = !kWide;   // packed per-lane argmax keys (int32 scores)
+    // Builds with at least 128 VGPRs per lane (4 waves per SIMD or fewer).
+    constexpr bool kRoomy = kRows || kSliceWavesPerEu<kBits, kMode, NT, kHbm, kRich> <= 4;
+    // Packed per-lane argmax keys (int32 scores); the other 8- and 16-wave
+    // builds keep the (score, block) loop, which spills nothing.
+    constexpr bool kPackable = !kWide && (NT <= 256 || kRoomy);
     // The per-group argmax folds packed (score, ~id) keys for int32 scores
     // (fewer dependent steps than (score, id) pairs).
     constexpr bool kKeyFold = !kWide;
+    // One straight-line key fold per group count where the register budget
+    // allows it.
+    constexpr bool kFoldPerP = kRoomy;
     static_assert(!kRows || (!kHbm && (kRich || NT >= 512)),
                   "rows in LDS: state in LDS, register-rich 2- and 4-wave tiles or 8- and 16-wave tiles");
     static_assert(!kTab || kBits == 8, "the score table needs 8-bit words (field < 128)");
@@ -990,7 +997,11 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                 for (int j = 0; j < 8; ++j) { accS[j] = kNoScore; accV[j] = -1; }
                 // int32 scores: the per-group argmax folds packed 64-bit keys
                 // across lanes and publishes one per group and wave.
-                auto foldKeys = [&](uint64_t (&kk)[8]) {
+                // Builds with registers to spare run one straight-line fold per
+                // group count; the lean 80-VGPR builds one fold with the steps
+                // chosen at run time (the ten straight-line copies made them
+                // spill to scratch).
+                auto foldKeysP = [&](uint64_t (&kk)[8]) {
                     // One straight-line reduction per group count (P a power
                     // of two up to 512): no branches between the steps.
                     auto foldP = [&](auto lgTag) {
@@ -1063,6 +1074,70 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
                         case 256: foldP(std::integral_constant<int, 8>{}); break;
                         default: foldP(std::integral_constant<int, 9>{}); break;   // 512
                     }
+                };
+                auto foldKeysRt = [&](uint64_t (&kk)[8]) {
+                    if (P >= 8) {
+                        const int Q = P >> 3;
+                        int held = 8, base = 0;
+                        if (Q <= 32) {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) scatterStepKey<32>(kk[j], kk[j + 4], lane);
+                            held = 4;
+                            base += (lane & 32) ? 4 : 0;
+                        }
+                        if (Q <= 16) {
+#pragma unroll
+                            for (int j = 0; j < 2; ++j) scatterStepKey<16>(kk[j], kk[j + 2], lane);
+                            held = 2;
+                            base += (lane & 16) ? 2 : 0;
+                        }
+                        if (Q <= 8) {
+                            scatterStepKey<8>(kk[0], kk[1], lane);
+                            held = 1;
+                            base += (lane & 8) ? 1 : 0;
+                        }
+                        if (Q <= 4) foldLanesKey<4>(kk[0], lane);
+                        if (Q <= 2) foldLanesKey<2>(kk[0], lane);
+                        if (Q <= 1) foldLanesKey<1>(kk[0], lane);
+                        if ((lane & 7) < Q) {
+                            const int row = wave * pstride + 8 * (lane & (Q - 1)) + base;
+#pragma unroll
+                            for (int k = 0; k < 8; ++k)
+                                if (k < held) part[row + k] = kk[k];
+                        }
+                    } else {
+                        if (P <= 4) {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) keepMaxKey(kk[j], kk[j + 4]);
+                        }
+                        if (P <= 2) {
+#pragma unroll
+                            for (int j = 0; j < 2; ++j) keepMaxKey(kk[j], kk[j + 2]);
+                        }
+                        if (P == 1) keepMaxKey(kk[0], kk[1]);
+                        int base = 0;
+                        if (P == 4) {
+                            scatterStepKey<32>(kk[0], kk[2], lane);
+                            scatterStepKey<32>(kk[1], kk[3], lane);
+                            base += (lane & 32) ? 2 : 0;
+                            scatterStepKey<16>(kk[0], kk[1], lane);
+                            base += (lane & 16) ? 1 : 0;
+                        } else if (P == 2) {
+                            scatterStepKey<32>(kk[0], kk[1], lane);
+                            base += (lane & 32) ? 1 : 0;
+                        }
+                        if (P == 1) foldLanesKey<32>(kk[0], lane);
+                        if (P <= 2) foldLanesKey<16>(kk[0], lane);
+                        foldLanesKey<8>(kk[0], lane);
+                        foldLanesKey<4>(kk[0], lane);
+                        foldLanesKey<2>(kk[0], lane);
+                        foldLanesKey<1>(kk[0], lane);
+                        if ((lane & (63 >> (P >> 1))) == 0) part[wave * pstride + base] = kk[0];
+                    }
+                };
+                auto foldKeys = [&](uint64_t (&kk)[8]) {
+                    if constexpr (kFoldPerP) foldKeysP(kk);
+                    else foldKeysRt(kk);
                 };
                 // int32 scores on the lane path: the lane keys (packed, or
                 // (score, block) pairs when the packed bound fails) become

@@ -102,6 +102,17 @@ __device__ __forceinline__ PhiloxSweep philoxSweep(uint32_t sLo, uint32_t sHi, u
     p.k1 = k1;
     return p;
 }
+// A copy of a wave-uniform value in a vector register: the compiler keeps
+// uniform values in SGPRs, and the kernel's many uniform values overflow them
+// (spilled SGPRs live in VGPR lanes, and every use reloads them with a
+// v_readlane); values a VALU instruction reads as an operand cost nothing
+// extra in a VGPR.
+__device__ __forceinline__ uint32_t vgprCopy(uint32_t x) {
+    uint32_t v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+    return v;
+}
+
 // The round keys of rounds 3-10 that the per-block rounds read: (k1 + 2 W1)
 // and (k0 + r W0, k1 + r W1) for r = 3..9.  kVgpr holds them in vector
 // registers (the builds with registers to spare), which leaves their scalar
@@ -109,43 +120,43 @@ __device__ __forceinline__ PhiloxSweep philoxSweep(uint32_t sLo, uint32_t sHi, u
 struct PhiloxKeys {
     uint32_t k1r2;
     uint32_t k0r[7], k1r[7];
+    uint32_t m0, m1;
 };
 template <bool kVgpr>
 __device__ __forceinline__ PhiloxKeys philoxKeys(uint32_t k0, uint32_t k1) {
     PhiloxKeys K;
     K.k1r2 = k1 + 2u * kPhiloxW1;
+    K.m0 = kPhiloxM0;
+    K.m1 = kPhiloxM1;
 #pragma unroll
     for (uint32_t r = 3; r < 10; ++r) {
         K.k0r[r - 3] = k0 + r * kPhiloxW0;
         K.k1r[r - 3] = k1 + r * kPhiloxW1;
     }
     if constexpr (kVgpr) {
-        auto toV = [](uint32_t x) {
-            uint32_t v;
-            asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
-            return v;
-        };
-        K.k1r2 = toV(K.k1r2);
+        K.k1r2 = vgprCopy(K.k1r2);
+        K.m0 = vgprCopy(K.m0);
+        K.m1 = vgprCopy(K.m1);
 #pragma unroll
         for (int r = 0; r < 7; ++r) {
-            K.k0r[r] = toV(K.k0r[r]);
-            K.k1r[r] = toV(K.k1r[r]);
+            K.k0r[r] = vgprCopy(K.k0r[r]);
+            K.k1r[r] = vgprCopy(K.k1r[r]);
         }
     }
     return K;
 }
 __device__ __forceinline__ void philoxBlock(const PhiloxSweep& s, const PhiloxKeys& K, uint32_t b,
                                             uint32_t (&o)[4]) {
-    const uint64_t p1 = static_cast<uint64_t>(kPhiloxM1) * b;         // round 1
+    const uint64_t p1 = static_cast<uint64_t>(K.m1) * b;         // round 1
     uint32_t c0 = static_cast<uint32_t>(p1 >> 32) ^ s.x1;
     uint32_t c1 = static_cast<uint32_t>(p1);
-    const uint64_t p0 = static_cast<uint64_t>(kPhiloxM0) * c0;        // round 2
+    const uint64_t p0 = static_cast<uint64_t>(K.m0) * c0;        // round 2
     c0 = c1 ^ s.u2a;
     uint32_t c2 = static_cast<uint32_t>(p0 >> 32) ^ s.u2b;
     uint32_t c3 = static_cast<uint32_t>(p0);
     {                                                                  // round 3
-        const uint64_t a = static_cast<uint64_t>(kPhiloxM0) * c0;
-        const uint64_t d = static_cast<uint64_t>(kPhiloxM1) * c2;
+        const uint64_t a = static_cast<uint64_t>(K.m0) * c0;
+        const uint64_t d = static_cast<uint64_t>(K.m1) * c2;
         c0 = static_cast<uint32_t>(d >> 32) ^ s.u3a;
         c2 = xor3(static_cast<uint32_t>(a >> 32), c3, K.k1r2);
         c1 = static_cast<uint32_t>(d);
@@ -153,8 +164,8 @@ __device__ __forceinline__ void philoxBlock(const PhiloxSweep& s, const PhiloxKe
     }
 #pragma unroll
     for (uint32_t r = 3; r < 10; ++r) {
-        const uint64_t a = static_cast<uint64_t>(kPhiloxM0) * c0;
-        const uint64_t d = static_cast<uint64_t>(kPhiloxM1) * c2;
+        const uint64_t a = static_cast<uint64_t>(K.m0) * c0;
+        const uint64_t d = static_cast<uint64_t>(K.m1) * c2;
         const uint32_t n0 = xor3(static_cast<uint32_t>(d >> 32), c1, K.k0r[r - 3]);
         const uint32_t n2 = xor3(static_cast<uint32_t>(a >> 32), c3, K.k1r[r - 3]);
         c1 = static_cast<uint32_t>(d);
@@ -636,6 +647,10 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     // The per-group argmax folds packed (score, ~id) keys for int32 scores
     // (fewer dependent steps than (score, id) pairs).
     constexpr bool kKeyFold = !kWide;
+    // The register-rich and rows-in-LDS builds hold the scoring loop's
+    // uniform operands (Philox multipliers, round keys and per-sweep terms)
+    // in vector registers, which relieves the scalar registers.
+    constexpr bool kVgprUniform = kRich || kRows;
     // One straight-line key fold per group count where the register budget
     // allows it.
     constexpr bool kFoldPerP = kRoomy;
@@ -724,7 +739,7 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
         const int nPhases = uni(it.nPhases);
         const uint32_t k0 = uni(static_cast<uint32_t>(it.key));
         const uint32_t k1 = uni(static_cast<uint32_t>(it.key >> 32));
-        const PhiloxKeys pkeys = philoxKeys<kRich || kRows>(k0, k1);
+        const PhiloxKeys pkeys = philoxKeys<kVgprUniform>(k0, k1);
 
         // ---- the chain's state: copied into LDS, or used in place ----
         const int wq = static_cast<int>(wordBytes(n, kBits) / 16);
@@ -864,7 +879,13 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
             const uint64_t s1 = step + 1;   // 1-based sweep id, the Philox counter
             const uint32_t sLo = static_cast<uint32_t>(s1);
             const uint32_t sHi = static_cast<uint32_t>(s1 >> 32);
-            const PhiloxSweep phx = philoxSweep(sLo, sHi, k0, k1);
+            PhiloxSweep phx = philoxSweep(sLo, sHi, k0, k1);
+            if constexpr (kVgprUniform) {
+                phx.x1 = vgprCopy(phx.x1);
+                phx.u2a = vgprCopy(phx.u2a);
+                phx.u2b = vgprCopy(phx.u2b);
+                phx.u3a = vgprCopy(phx.u3a);
+            }
             // Packed per-lane keys when every score of the sweep satisfies
             // |score| <= |beta| * deltaMax + kGumbelMaxMag < 2^(31 - lanesBits)
             // (the (score, block) loop otherwise).

@@ -89,6 +89,37 @@ def test_general_group_counts(P):
     _case(f"P={P}", csr, [5, 6, 7], _log_tables([40.0, 60.0, 80.0], iters), phases, iters)
 
 
+@pytest.mark.parametrize("threads", [128, 512])
+def test_u16_words_lane_summed_winner_update(threads):
+    # weighted degree about 10^4: 16-bit words, where 3B exceeds the field's
+    # range, so FLIP sums each winner's couplings over its lanes and adds them
+    # in one update instead of one atomic per adjacent winner
+    rng = np.random.default_rng(100)
+    n, m = 509, 1800
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    keep = u != v
+    w = rng.integers(1, 1200, m) * np.where(rng.random(m) < 0.5, -1, 1)
+    csr = R.graph_from_edges(u[keep], v[keep], w[keep])
+    B = R.max_weighted_degree(csr)
+    assert 16383 // 3 < B <= 16383, B
+    phases, iters = [64, 8, 1], [60, 60, 60]
+    seeds = [21, 22, 23]
+    init = np.stack([R.generate_random_spins(s, csr.n) for s in seeds])
+    solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+    beta = _log_tables([3000.0, 5000.0, 8000.0], iters)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        job = R.Job(csr, init, solver, beta, phases, iters)
+        ctx.batch_run([(gid, job)], threads=threads)
+        geo = ctx.last_launch()
+    print(f"[gpu-parity] u16 lane-summed: B={B} threads={threads} {geo}", flush=True)
+    assert geo["word_bits"] == 16 and geo["threads_per_chain"] == threads, geo
+    end, best, cv, bv = run_oracle_chains(csr, init, solver, beta, phases, iters)
+    assert np.array_equal(job.end_spins, end) and np.array_equal(job.best_spins, best)
+    assert np.array_equal(job.final_vals, cv) and np.array_equal(job.best_vals, bv)
+
+
 def test_wide_words_u32():
     # a weighted degree above 16383 forces 32-bit state words
     rng = np.random.default_rng(11)

@@ -318,7 +318,7 @@ def test_header_is_c_and_struct_layouts_match_ctypes(tmp_path):
 def test_group_map_reciprocal_needs_one_correction():
     """The kernel's x mod P for P not a power of two: q = mulhi(x, m) with
     m = floor((2^32 - 1) / P) is floor(x / P) or one less for every x < 2^31,
-    so r = x - q * P needs at most one subtraction of P (anneal_kernels.hip,
+    so r = x - q * P needs at most one subtraction of P (anneal_slice.hpp,
     modBy).  Checked for every P up to 4096 on random and edge values of x."""
     rng = np.random.default_rng(7)
     xs = np.concatenate([rng.integers(0, 2**31, 20000, dtype=np.int64),
@@ -336,7 +336,7 @@ def test_group_map_reciprocal_needs_one_correction():
 
 
 def test_packed_lane_key_order_and_decode():
-    """The scoring loop's per-lane key (anneal_kernels.hip, scorePackedLanes):
+    """The scoring loop's per-lane key (anneal_slice.hpp, scorePackedLanes):
     with the table variant's score t = beta * (delta + 64) + gumbel, the key
     ((t << k) + (K - 1 - i) - ((64 * beta) << k)) mod 2^32, read as int32,
     equals s * 2^k + (K - 1 - i) for the true score s whenever
@@ -367,3 +367,38 @@ def test_packed_lane_key_order_and_decode():
             best = max(items, key=lambda e: e[2])
             want = max(items, key=lambda e: (e[0], -e[1]))
             assert best[:2] == want[:2]
+
+
+def test_lane_keys_fold_as_64_bit_keys():
+    """The cross-lane fold (anneal_slice.hpp, the packed branch of select and
+    RESOLVE): each lane slot's int32 key s * 2^k + (K - 1 - i) becomes the u64
+    key ((key ^ 2^31) << 32) | ~(8 tid + j).  Over the slots of every lane of a
+    group, the unsigned maximum is the vertex of largest score with the
+    smallest id, where vertex = 8 (tid + i * threads) + j, and RESOLVE's
+    decode ~lo + ((K - 1 - (hi & (2^k - 1))) << (3 + log2 threads)) returns
+    that vertex.  Scores are drawn from a narrow range so that ties across
+    lanes, slots and blocks are common."""
+    rng = np.random.default_rng(12)
+    for lg_nt in (7, 8, 9, 10):
+        nt = 1 << lg_nt
+        for K in (1, 2, 3, 5, 10):
+            k = 0 if K <= 1 else (K - 1).bit_length()
+            for _ in range(40):
+                cands = []
+                for _c in range(64):
+                    tid = int(rng.integers(0, nt))
+                    j = int(rng.integers(0, 8))
+                    i = int(rng.integers(0, K))
+                    s = int(rng.integers(-3, 4)) * 1000 + int(rng.integers(-2, 3))
+                    key32 = s * 2**k + (K - 1 - i)
+                    hi = (key32 % 2**32) ^ 0x80000000
+                    lo = ~(8 * tid + j) & 0xFFFFFFFF
+                    vertex = 8 * (tid + i * nt) + j
+                    cands.append((s, vertex, (hi << 32) | lo))
+                key = max(c[2] for c in cands)
+                s_want, v_want = max((c[0], -c[1]) for c in cands)
+                hi, lo = key >> 32, key & 0xFFFFFFFF
+                mask = (1 << k) - 1
+                v = (~lo & 0xFFFFFFFF) + ((K - 1 - (hi & mask)) << (3 + lg_nt))
+                assert v == -v_want, (nt, K, key)
+                assert hi != 0   # no real key is the empty slot's 0

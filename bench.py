@@ -120,6 +120,11 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=96, help="serve: number of requests")
     ap.add_argument("--slice", type=int, default=500, help="serve: sweeps per serving slice")
     ap.add_argument("--no-plain", action="store_true", help="serve: served leg only (no plain FIFO comparison)")
+    ap.add_argument("--serve-sweeps", default="10000,100000", metavar="LO,HI",
+                    help="serve: sweeps per request, log-uniform in [LO, HI]")
+    ap.add_argument("--plain-stride", type=int, default=1, metavar="S",
+                    help="batch: run the plain one-graph-per-launch leg on every S-th graph only (its rate "
+                         "is per graph and sweep, so a stratified sample measures it at full length)")
     ap.add_argument("--graphs", type=int, default=256, help="batch: number of graphs")
     ap.add_argument("--batch-sweeps", type=int, default=20000, help="batch: sweeps per chain")
     ap.add_argument("--gset-sweeps", type=int, default=100000, help="gset: sweeps per chain")
@@ -378,12 +383,14 @@ def bench_sparse(args) -> int:
     return rc
 
 
-def run_batch(R, graphs_count: int, sweeps: int, threads: int = 0) -> dict:
+def run_batch(R, graphs_count: int, sweeps: int, threads: int = 0, plain_stride: int = 1) -> dict:
     """Batched mixed instances: `graphs_count` independent graphs (n
     log-uniform in [800, 20000], family uniform over ER mean degree 6 / torus
     +/-1 / random degree 2-6 +/-1), 64 chains each, in one hipGraph-captured
     persistent launch, against the plain mode: one launch per graph, graphs
-    one after another.  Every chain must be identical in both modes."""
+    one after another.  Every chain must be identical in both modes.  With
+    plain_stride S > 1 the plain leg runs every S-th graph only, and its rate
+    is that sample's work over its time."""
     from rfmf_amd import workloads as W
     iters = [sweeps // 8] * 8
     t0 = time.perf_counter()
@@ -391,33 +398,43 @@ def run_batch(R, graphs_count: int, sweeps: int, threads: int = 0) -> dict:
     log(f"[batch] generated {len(graphs)} graphs in {time.perf_counter() - t0:.1f} s; "
         f"sum n = {sum(g.n for _, g in graphs)}")
     jobs_b = [_job(R, g, range(64 * i + 1, 64 * i + 65), iters) for i, (_, g) in enumerate(graphs)]
-    jobs_p = [_job(R, g, range(64 * i + 1, 64 * i + 65), iters) for i, (_, g) in enumerate(graphs)]
+    sample = list(range(0, len(graphs), max(1, plain_stride)))
+    jobs_p = {i: _job(R, graphs[i][1], range(64 * i + 1, 64 * i + 65), iters) for i in sample}
     work = 64 * len(graphs) * sum(iters)
+    work_p = 64 * len(sample) * sum(iters)
     with R.Context(0) as ctx:
         gids = [ctx.upload_graph(g) for _, g in graphs]
         ms_b = ctx.batch_run(list(zip(gids, jobs_b)), slice_sweeps=sum(iters), use_hip_graph=True,
                              autotune=threads == 0, threads=threads)
         geo = ctx.last_launch()
+        log(f"[batch] batched {ms_b:.1f} ms; plain leg on {len(sample)} of {len(graphs)} graphs")
         ms_p = 0.0
-        for gid, j in zip(gids, jobs_p):
-            ms_p += ctx.batch_run([(gid, j)])
-    same = all(np.array_equal(a.end_spins, b.end_spins) and np.array_equal(a.best_spins, b.best_spins) and
-               np.array_equal(a.best_vals, b.best_vals) for a, b in zip(jobs_b, jobs_p))
-    rb, rp = work / (ms_b / 1e3), work / (ms_p / 1e3)
+        for k, i in enumerate(sample):
+            ms_p += ctx.batch_run([(gids[i], jobs_p[i])])
+            if (k + 1) % 16 == 0:
+                log(f"[batch] plain {k + 1}/{len(sample)} graphs, {ms_p / 1e3:.1f} s of kernels")
+    same = all(np.array_equal(jobs_b[i].end_spins, j.end_spins) and
+               np.array_equal(jobs_b[i].best_spins, j.best_spins) and
+               np.array_equal(jobs_b[i].best_vals, j.best_vals) for i, j in jobs_p.items())
+    rb, rp = work / (ms_b / 1e3), work_p / (ms_p / 1e3)
     med_b = float(np.median(np.concatenate([j.best_vals for j in jobs_b])))
-    med_p = float(np.median(np.concatenate([j.best_vals for j in jobs_p])))
+    med_p = float(np.median(np.concatenate([j.best_vals for j in jobs_p.values()])))
+    med_bs = float(np.median(np.concatenate([jobs_b[i].best_vals for i in sample])))
     log(f"[batch] batched {ms_b:.1f} ms {rb:.3e} chain-steps/s ({geo}); plain {ms_p:.1f} ms {rp:.3e}; "
         f"x{rb / rp:.2f}; identical: {same}")
-    return {"config": f"batch_{len(graphs)}graphs_64chains", "sweeps": sum(iters), "chains": 64 * len(graphs),
-            "batched_value": rb, "plain_value": rp, "speedup": rb / rp, "batched_kernel_ms": ms_b,
-            "plain_kernel_ms": ms_p, "geometry": geo, "identical_results": bool(same),
-            "best_cut_median_batched": med_b, "best_cut_median_plain": med_p}
+    out = {"config": f"batch_{len(graphs)}graphs_64chains", "sweeps": sum(iters), "chains": 64 * len(graphs),
+           "batched_value": rb, "plain_value": rp, "speedup": rb / rp, "batched_kernel_ms": ms_b,
+           "plain_kernel_ms": ms_p, "geometry": geo, "identical_results": bool(same),
+           "best_cut_median_batched": med_b, "best_cut_median_plain": med_p}
+    if len(sample) < len(graphs):
+        out.update({"plain_graphs": len(sample), "best_cut_median_batched_same_graphs": med_bs})
+    return out
 
 
 def bench_batch(args) -> int:
     import rfmf_amd as R
     sweeps = args.steps * args.sweeps_per_step if args.steps != 20 else args.batch_sweeps
-    r = run_batch(R, args.graphs, sweeps, args.threads)
+    r = run_batch(R, args.graphs, sweeps, args.threads, args.plain_stride)
     _emit({"metric": "chain-steps/sec", **r})
     return 0 if r["identical_results"] else 1
 
@@ -453,6 +470,8 @@ def run_serve(R, nreq: int, qps: float, slice_sweeps: int, lo: int = 10_000, hi:
             ctx.batch_run([(gid, j)])
             ctx.release_graph(gid)
             finish.append(time.perf_counter() - t0)
+            if (i + 1) % 4 == 0:
+                log(f"[serve] plain {i + 1}/{nreq} requests done at {finish[-1]:.1f} s")
         if plain_leg:
             lat_p = np.array(finish) - arrivals
             span_p = finish[-1] - arrivals[0]
@@ -467,8 +486,10 @@ def run_serve(R, nreq: int, qps: float, slice_sweeps: int, lo: int = 10_000, hi:
                 if delay > 0:
                     time.sleep(delay)
                 ids.append(srv.submit(j))
-            for rid in ids:
+            for k, rid in enumerate(ids):
                 srv.wait(rid, 1200.0)
+                if (k + 1) % 4 == 0:
+                    log(f"[serve] served {k + 1}/{nreq} requests done at {time.perf_counter() - t0:.1f} s")
             polls = [srv.poll(rid) for rid in ids]
             st = srv.stats()
     lat_s = np.array([td - ts for _, ts, _, td in polls])
@@ -491,7 +512,8 @@ def run_serve(R, nreq: int, qps: float, slice_sweeps: int, lo: int = 10_000, hi:
 
 def bench_serve(args) -> int:
     import rfmf_amd as R
-    r = run_serve(R, args.requests, args.qps, args.slice, plain_leg=not args.no_plain)
+    lo, hi = (int(x) for x in args.serve_sweeps.split(","))
+    r = run_serve(R, args.requests, args.qps, args.slice, lo, hi, plain_leg=not args.no_plain)
     _emit({"metric": "chain-steps/sec and solve latency", **r})
     return 0 if r["identical_results"] or args.no_plain else 1
 

@@ -274,11 +274,26 @@ template <int kBits> struct Block8 {
         }
     }
     // Spin bits of the 8 words as one byte (bit j = vertex 8b+j is +1).
+    // Bits above 7 of the result are not specified.
     __device__ __forceinline__ uint32_t spinByte() const {
-        uint32_t m = 0;
+        if constexpr (kBits == 8) {
+            // Spins at bit 7 of each byte: s holds word i's at bit 8i and word
+            // i+4's at 8i+4 (i < 4); shifting s by 7i moves both to bits i and
+            // i+4, and no two of the OR's sources meet below bit 8.
+            const uint32_t s = ((d[0] >> 7) & 0x01010101u) | ((d[1] >> 3) & 0x10101010u);
+            return s | (s >> 7) | (s >> 14) | (s >> 21);
+        } else if constexpr (kBits == 16) {
+            // Spins at bits 15 and 31 of each dword: dword q's land at bits 2q
+            // and 16 + 2q of s, and s >> 15 moves the second to 2q + 1.
+            const uint32_t s = ((d[0] >> 15) & 0x00010001u) | ((d[1] >> 13) & 0x00040004u) |
+                               ((d[2] >> 11) & 0x00100010u) | ((d[3] >> 9) & 0x00400040u);
+            return s | (s >> 15);
+        } else {
+            uint32_t m = 0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) m |= ((word(j) >> (kBits - 1)) & 1u) << j;
-        return m;
+            for (int j = 0; j < 8; ++j) m |= ((word(j) >> (kBits - 1)) & 1u) << j;
+            return m;
+        }
     }
 };
 
@@ -831,13 +846,14 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
         // Best-spin snapshot: the spins after an improving sweep, taken before
         // the next sweep's flips (and at the end of the slice).  Each thread
         // packs its own blocks; nothing else touches bl8 until the copy-out.
+        // The last block needs no mask: the words past n are zero (written so
+        // by initWordGroup, and no winner or row entry ever names them), so
+        // their spin bits are clear.
         auto snapshot = [&]() {
             for (int b = tid; b < nBlk; b += NT) {
                 Block8<kBits> blk;
                 blk.template load<kHbm>(wl, b);
-                uint32_t m = blk.spinByte();
-                if (8 * b + 8 > n) m &= (1u << (n - 8 * b)) - 1u;
-                storeAt<kHbm>(bl8 + b, static_cast<uint8_t>(m));
+                storeAt<kHbm>(bl8 + b, static_cast<uint8_t>(blk.spinByte()));
             }
         };
 

@@ -402,3 +402,29 @@ def test_lane_keys_fold_as_64_bit_keys():
                 v = (~lo & 0xFFFFFFFF) + ((K - 1 - (hi & mask)) << (3 + lg_nt))
                 assert v == -v_want, (nt, K, key)
                 assert hi != 0   # no real key is the empty slot's 0
+
+
+def test_spin_byte_gathers_the_eight_spin_bits():
+    """The best-spin snapshot's gather (anneal_slice.hpp, Block8::spinByte):
+    for 8-bit words s = ((d0 >> 7) & 0x01010101) | ((d1 >> 3) & 0x10101010)
+    and s | s >> 7 | s >> 14 | s >> 21, for 16-bit words s = OR over dwords q
+    of (dq >> (15 - 2q)) & (0x10001 << 2q) and s | s >> 15: the low byte is
+    the spin (top) bit of word j at bit j, for random words and the all-set
+    and all-clear cases."""
+    rng = np.random.default_rng(5)
+    d = rng.integers(0, 2**32, (20000, 4), dtype=np.uint64)
+    d = np.concatenate([d, np.zeros((1, 4), np.uint64), np.full((1, 4), 2**32 - 1, np.uint64)])
+    s8 = ((d[:, 0] >> 7) & 0x01010101) | ((d[:, 1] >> 3) & 0x10101010)
+    got8 = (s8 | (s8 >> 7) | (s8 >> 14) | (s8 >> 21)) & 0xFF
+    s16 = np.zeros(len(d), np.uint64)
+    for q in range(4):
+        s16 |= (d[:, q] >> np.uint64(15 - 2 * q)) & np.uint64(0x00010001 << (2 * q))
+    got16 = (s16 | (s16 >> 15)) & 0xFF
+    want8 = np.zeros(len(d), np.uint64)
+    want16 = np.zeros(len(d), np.uint64)
+    for j in range(8):
+        want8 |= ((d[:, j // 4] >> np.uint64(8 * (j % 4) + 7)) & 1) << np.uint64(j)
+        want16 |= ((d[:, j // 2] >> np.uint64(16 * (j % 2) + 15)) & 1) << np.uint64(j)
+    assert np.array_equal(got8, want8)
+    assert np.array_equal(got16, want16)
+    assert got8[-1] == 0xFF and got8[-2] == 0 and got16[-1] == 0xFF and got16[-2] == 0

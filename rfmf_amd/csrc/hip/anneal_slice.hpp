@@ -15,7 +15,7 @@
 //   * one workgroup of 128-1024 threads (2-16 wave64) anneals one chain at a
 //     time, and the chain's whole state lives in LDS for the length of a
 //     slice: one packed word per vertex holding the spin bit and delta =
-//     spin*S (biased to stay non-negative; 8/16/32 bits chosen from the
+//     spin*S (biased to stay non-negative; 8/16/32/64 bits chosen from the
 //     graph's max weighted degree) plus a best-spin bitmask;
 //   * scoring is one v_mad_i32_i24 per vertex: the bias is folded either into
 //     a per-sweep copy of the Gumbel LUT (LUT - bias*beta, wrapping mod 2^32;
@@ -27,17 +27,20 @@
 //   * Philox's block-independent half of rounds 1-3 is computed once per
 //     sweep on the scalar unit;
 //   * the per-group argmax is kept in registers (each lane's 8 vertex slots
-//     map to fixed groups when P divides 8*threads; in 2- and 4-wave tiles as
-//     one packed int32 key per slot, score and block number, kept with one
-//     v_max_i32), combined across lanes by a reduce-scatter on DPP and
+//     map to fixed groups when P divides 8*threads; for int32 scores as one
+//     packed key per slot, score and block number, kept with one v_max_i32,
+//     in every tile with the registers for it), combined across lanes as
+//     64-bit (key, ~slot) keys by a reduce-scatter on DPP and
 //     v_permlane{16,32}_swap, and across waves through LDS: no atomics on the
 //     hot path;
 //   * the winner map for corr[] is implicit: vertex x can only win group
 //     x mod P, so "x is a winner" is pick[x mod P] == x and no n-sized
 //     winnerGroup array exists;
 //   * FLIP updates non-winner neighbours in delta space with 32-bit LDS
-//     atomics on the packed words, and each winner's own word once, with its
-//     flip and every adjacent winner's coupling summed first;
+//     atomics on the packed words; a winner's own word takes its flip in one
+//     add and each adjacent winner's coupling in one more (or, where such
+//     partial sums could leave the field, the couplings summed over the
+//     winner's lanes first);
 //   * the kernel is persistent over a work list of (graph, chain) items and
 //     runs each item for a slice of sweeps, saving the state to HBM at the end,
 //     so the same kernel serves single-graph runs, multi-graph batches and the

@@ -583,9 +583,10 @@ void shapeGroup(Context* ctx, const hipDeviceProp_t& props, GroupShape& g, int64
     }
     // The 24-bit score variants' 2- and 4-wave tiles have a register-rich
     // build (4 workgroups per CU instead of 6): taken when it keeps every
-    // chain resident that the lean one would.
+    // chain resident that the lean one would (and the launch has no other
+    // group, Context::planGroups).
     g.rich = false;
-    if ((g.mode == dev::kScoreTab || g.mode == dev::kScore24) && !g.hbm && g.threads <= 256) {
+    if ((g.mode == dev::kScoreTab || g.mode == dev::kScore24) && !g.hbm && g.threads <= 256 && !knobs.noRich) {
         const int occRich = occupancy(g.bits, g.mode, g.threads, g.plan.total, false, true);
         if (occRich > 0 && residentOf(occRich) >= residentOf(occ)) {
             g.rich = true;
@@ -684,6 +685,18 @@ std::vector<Context::LaunchGroup> Context::planGroups(const std::vector<Resident
             at = stop;
         }
         for (LaunchGroup& c : classes) out.push_back(std::move(c));
+    }
+    // The register-rich build only for a launch of one group.  Its waves hold
+    // up to 128 VGPRs, so beside three of its workgroups per CU a concurrent
+    // group's 8-wave workgroup (two 80-VGPR waves per SIMD) finds no room,
+    // and that group runs on the CUs left over; the lean build leaves it room
+    // (the 960 chains of the 12 Gset instances in one batch: the 16-bit
+    // group, which ends last, 17% sooner; the 256-graph batch 3% faster).
+    if (out.size() > 1) {
+        LaunchKnobs lean = knobs;
+        lean.noRich = true;
+        for (LaunchGroup& g : out)
+            if (g.rich) finishGroup(g, g.items.front().n, g.maxP, lean);
     }
     // Largest work first: its launch starts first, and the others, on
     // concurrent lanes, fill its tail.
@@ -854,6 +867,9 @@ void Context::autotune(const std::vector<ResidentJob*>& jobs, LaunchKnobs* knobs
     uint64_t slice = knobs->sliceSweeps > 0 ? static_cast<uint64_t>(knobs->sliceSweeps) : ~uint64_t{0};
     if (budget > 0) slice = std::min(slice, budget);
     const std::vector<LaunchGroup> groups = planGroups(jobs, plain, slice);
+    // Each group's trials run it alone; with other groups in the launch they
+    // time the build the launch will run (planGroups: no register-rich build).
+    if (groups.size() > 1) plain.noRich = true;
     knobs->groupThreads.clear();
     last_.tuned = true;
     for (size_t gi = 0; gi < groups.size(); ++gi) {

@@ -170,6 +170,9 @@ struct LaunchKnobs {
     int scoreTable = 0;        // 8-bit words: 0 table when it costs no resident chains, 1 always, -1 never
     int rowsInLds = 0;         // compact rows in LDS: 0 when they cost no resident chains, 1 whenever they fit, -1 never
     int stateInHbm = 0;        // 0: only chains whose state exceeds LDS keep it in HBM; 1: every chain
+    // Set by the planner, not by callers: the register-rich build is not
+    // taken (a launch of several groups, and the autotuner's trials for one).
+    bool noRich = false;
     // Tiles the autotuner chose per launch group (Context::groupKey); a group
     // not listed uses `threads`, else the cached choice or the heuristic.
     std::vector<std::pair<uint64_t, int>> groupThreads;

@@ -725,11 +725,6 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     int64_t* dpart = reinterpret_cast<int64_t*>(lds + plan.dpartOff);
     int32_t* misc = reinterpret_cast<int32_t*>(lds + plan.miscOff);
 
-    // Register-rich launches hold all their chains resident at a few per CU,
-    // so each chain's sweep latency is their critical path; a higher wave
-    // priority lets them issue first beside a concurrent lean group's waves
-    // (the size classes of a batch share the CUs).
-    if constexpr (kRich) __builtin_amdgcn_s_setprio(2);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform per wave (an SGPR)

@@ -223,6 +223,21 @@ __device__ __forceinline__ void atomicAddAt(uint32_t* p, uint32_t v) {
 }
 
 template <bool kGlobal>
+__device__ __forceinline__ void atomicOrAt(uint32_t* p, uint32_t v) {
+    if constexpr (kGlobal)
+        __hip_atomic_fetch_or(globalIn(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        atomicOr(p, v);
+}
+template <bool kGlobal>
+__device__ __forceinline__ void atomicAndAt(uint32_t* p, uint32_t v) {
+    if constexpr (kGlobal)
+        __hip_atomic_fetch_and(globalIn(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        atomicAnd(p, v);
+}
+
+template <bool kGlobal>
 __device__ __forceinline__ void atomicAddAt(uint64_t* p, uint64_t v) {
     if constexpr (kGlobal)
         __hip_atomic_fetch_add(globalIn(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -672,6 +687,12 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
     // One straight-line key fold per group count where the register budget
     // allows it.
     constexpr bool kFoldPerP = kRoomy;
+    // The register-rich builds without rows in LDS let the best-spin snapshot
+    // wait for the end of a run of improving sweeps (below): the 1024-chain
+    // torus gains 2%, the 80-chain G81 1%.  The rows-in-LDS builds' latency-
+    // bound sweeps (one chain per CU) lost 1-2% to it, and the lean builds'
+    // batches 1%, so they pack the state before the next sweep flips.
+    constexpr bool kDeferBest = kRich && !kRows;
     static_assert(!kRows || (!kHbm && (kRich || NT >= 512)),
                   "rows in LDS: state in LDS, register-rich 2- and 4-wave tiles or 8- and 16-wave tiles");
     static_assert(!kTab || kBits == 8, "the score table needs 8-bit words (field < 128)");
@@ -841,22 +862,57 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
             const int lg = half <= 1 ? 0 : 32 - __clz(half - 1);
             return lg > 6 ? 6 : lg;
         }();
-        // Best-spin snapshot: the spins after an improving sweep, taken before
-        // the next sweep's flips (and at the end of the slice).  Each thread
-        // packs its own blocks; nothing else touches bl8 until the copy-out.
-        // The last block needs no mask: the words past n are zero (written so
-        // by initWordGroup, and no winner or row entry ever names them), so
-        // their spin bits are clear.
+        // Best-spin snapshot: the spins of the words now into bl.  Each thread
+        // packs its own blocks (bytes of bl).  The last block's bits past n
+        // are zero without a mask (initWordGroup writes the words past n as
+        // zero, and no winner or row entry names them).  The 80-VGPR builds
+        // apply one anyway, since without it their registers spill to scratch,
+        // and so do the deferring builds, whose schedule it suits (the
+        // 1024-chain torus 1.6% faster with it than without).
         auto snapshot = [&]() {
             for (int b = tid; b < nBlk; b += NT) {
                 Block8<kBits> blk;
                 blk.template load<kHbm>(wl, b);
-                storeAt<kHbm>(bl8 + b, static_cast<uint8_t>(blk.spinByte()));
+                uint32_t m = blk.spinByte();
+                if constexpr (!kRoomy || kDeferBest)
+                    if (8 * b + 8 > n) m &= (1u << (n - 8 * b)) - 1u;
+                storeAt<kHbm>(bl8 + b, static_cast<uint8_t>(m));
             }
         };
 
+        // kDeferBest: a sweep that raises best leaves the best state in the
+        // words; it is packed only when the next sweep fails to raise best
+        // again, from the words then, with that sweep's winners set back to
+        // their pre-sweep spins (their records stay in pks until the next
+        // RESOLVE): a run of improving sweeps (a descent) costs one pass.
+        // `pending`: the state before the last sweep raised best and is not in
+        // bl.  Otherwise the state is packed before the next sweep flips.  At a
+        // slice's start bl already holds the best state.
+        auto snapshotBeforeLast = [&]() {
+            snapshot();
+            __syncthreads();   // every byte of bl written
+            for (int g = tid; g < P; g += NT) {
+                const i32x2 rec = pks[g];
+                if (rec.x >= 0) {
+                    const uint32_t bit = 1u << (rec.x & 31);
+                    if (rec.y > 0)
+                        atomicOrAt<kHbm>(bl + (rec.x >> 5), bit);
+                    else
+                        atomicAndAt<kHbm>(bl + (rec.x >> 5), ~bit);
+                }
+            }
+        };
+        const uint64_t stepStart = step;
+        bool pending = false;
+
         while (step < stepEnd) {
-            if (improved) snapshot();
+            if constexpr (kDeferBest) {
+                // P is still the last sweep's here (the phase advances below).
+                if (pending && !improved) snapshotBeforeLast();
+                pending = improved != 0 && step != stepStart;
+            } else {
+                if (improved) snapshot();
+            }
             if (step >= phaseStop && ph < nPhases - 1) {
                 while (ph < nPhases - 1 && uni(phaseEnd[ph]) <= step) ++ph;
                 phaseStop = uni(phaseEnd[ph]);
@@ -1531,8 +1587,15 @@ void annealSlice(const ChainItem* __restrict__ items, int nItems, uint32_t* __re
             ++step;
         }
 
-        // An improving last sweep of the slice.
-        if (improved) snapshot();
+        // The slice's best state: the last sweep's, or the one before it.
+        if constexpr (kDeferBest) {
+            if (improved && step != stepStart)
+                snapshot();
+            else if (pending && !improved)
+                snapshotBeforeLast();
+        } else {
+            if (improved) snapshot();
+        }
         __syncthreads();
         if constexpr (!kHbm) {
             GlobalOf<u32x4>* gw = globalIn(reinterpret_cast<u32x4*>(it.words));

@@ -295,6 +295,31 @@ def test_score_table_lean_and_rich_builds():
     assert np.array_equal(small.final_vals[:8], cv) and np.array_equal(small.best_vals[:8], bv)
 
 
+@pytest.mark.parametrize("slice_sweeps", [0, 1, 3, 37])
+def test_deferred_best_state_across_slices(slice_sweeps):
+    """The register-rich build without rows in LDS packs an improving sweep's
+    state only when the next sweep fails to improve, from the words with that
+    sweep's winners set back (at the end of a slice too).  A descent at high
+    beta (P = 8, then 1, most sweeps improving) run whole and in slices of 1,
+    3 and 37 sweeps gives the oracle's best and final states."""
+    csr = W.torus(40, 50, seed=3)   # n = 2000, 8-bit words, +/-1 weights
+    phases, iters = [8, 1], [150, 250]
+    seeds = list(range(21, 29))
+    init = np.stack([R.generate_random_spins(s, csr.n) for s in seeds])
+    solver = [(s + R.K_SOLVER_SEED_OFFSET) % 2**64 for s in seeds]
+    beta = R.build_linear_beta_table(1.5, 3.0, sum(iters), len(seeds))
+    job = R.Job(csr, init, solver, beta, phases, iters)
+    print(f"[batch] deferred best state, slice={slice_sweeps}", flush=True)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ctx.batch_run([(gid, job)], slice_sweeps=slice_sweeps, threads=256, score_table=1, rows_in_lds=-1)
+        geo = ctx.last_launch()
+    assert geo["wg_per_cu"] == 4 and not geo["rows_in_lds"], geo   # the register-rich build
+    end, best, cv, bv = run_oracle_chains(csr, init, solver, beta, phases, iters)
+    assert np.array_equal(job.end_spins, end) and np.array_equal(job.best_spins, best)
+    assert np.array_equal(job.final_vals, cv) and np.array_equal(job.best_vals, bv)
+
+
 def test_server_admission_passes_a_blocked_request_at_most_eight_times(graphs):
     # A long request holds most of the budget; a large request behind it
     # cannot fit until it retires; ten small requests arrive after the large

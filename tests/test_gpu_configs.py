@@ -175,3 +175,45 @@ def test_sparse_graphs_with_4096_chains_match_oracle(name, n, m, pm1):
         assert R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c]
         assert R.cut_value_csr(csr, job.end_spins[c]) == job.final_vals[c]
     oracle_check(job, list(range(0, 4096, 128)) + [4095], name)
+
+
+def test_best_cut_distribution_over_32_seeds_er800():
+    """The small configuration's graph (ER n=800, p=0.06, unit weights) with
+    32 seeds, run three ways: one plain launch (autotuned geometry), the same
+    32 chains inside a mixed batch beside two other graphs, and the CPU
+    oracle.  Each seed's best cut, final cut and partitions are equal in all
+    three, so the 32-seed best-cut distributions are equal (sorted vectors,
+    median, mean); the quality bar for batched against plain is a median
+    within 0.1%, and it holds at 0."""
+    csr = W.erdos_renyi(800, 0.06, seed=0)
+    sweeps = 16_000
+    iters = [sweeps // 8] * 8
+    seeds = range(1, 33)
+    plain = make_job(csr, seeds, iters)
+    mixed = make_job(csr, seeds, iters)
+    others = [(W.torus(40, 50, seed=5), range(101, 165)), (W.degree_random(3000, seed=6), range(201, 233))]
+    other_jobs = [make_job(g, s, iters) for g, s in others]
+    print(f"[32 seeds] ER n=800 p=0.06, 32 chains x {sweeps} sweeps: plain, in a 3-graph batch, oracle", flush=True)
+    with R.Context(0) as ctx:
+        gid = ctx.upload_graph(csr)
+        ctx.batch_run([(gid, plain)], autotune=True)
+        geo = ctx.last_launch()
+        oids = [ctx.upload_graph(g) for g, _ in others]
+        ctx.batch_run([(gid, mixed)] + list(zip(oids, other_jobs)), autotune=True)
+    print(f"[32 seeds] plain geometry {geo}", flush=True)
+    beta = R.build_linear_beta_table(0.1, 3.0, sweeps, 32)
+    end, best, cv, bv = run_oracle_chains(csr, plain.init, plain.seeds, beta, plain.phases, plain.iters)
+    for c in range(32):
+        assert np.array_equal(plain.best_spins[c], best[c]) and np.array_equal(plain.end_spins[c], end[c]), c
+        assert np.array_equal(mixed.best_spins[c], best[c]) and np.array_equal(mixed.end_spins[c], end[c]), c
+        assert R.cut_value_csr(csr, plain.best_spins[c]) == plain.best_vals[c] == bv[c], c
+        assert plain.final_vals[c] == mixed.final_vals[c] == cv[c], c
+    dist = {k: np.sort(np.asarray(v, dtype=np.int64)) for k, v in
+            (("plain", plain.best_vals), ("batched", mixed.best_vals), ("oracle", bv))}
+    assert np.array_equal(dist["plain"], dist["oracle"]) and np.array_equal(dist["batched"], dist["oracle"])
+    med_p, med_b = float(np.median(dist["plain"])), float(np.median(dist["batched"]))
+    assert abs(med_b - med_p) <= 0.001 * abs(med_p)
+    print(f"[32 seeds] best cuts min/median/mean/max {dist['plain'][0]}/{med_p}/{dist['plain'].mean():.2f}/"
+          f"{dist['plain'][-1]} in all three", flush=True)
+    for j, (g, _) in zip(other_jobs, others):
+        oracle_check(j, [0, j.chains - 1], f"batch neighbour n={g.n}")

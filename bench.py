@@ -105,6 +105,8 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=100)
     ap.add_argument("--cols", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-small-config", action="store_true",
+                    help="leave the ER n=800 GPU/CPU comparison out of the cpu_baseline leg")
     ap.add_argument("--no-north-star", action="store_true",
                     help="headline only: skip the batched and served legs")
     # CPU sample: 4 chains per host thread (at least 64) x 64000 sweeps, about
@@ -213,6 +215,10 @@ def main() -> int:
                          f"(the host's CPU share for this process; {visible} CPUs visible)"}
         log(f"[bench] cpu baseline {rate:.3e} chain-steps/s on {threads} threads of {visible} visible "
             f"({model}, {sec:.2f} s)")
+        if not args.no_small_config:
+            # The small configuration in full (10^6 sweeps of 64 chains) on
+            # both paths, partitions compared per seed.
+            cpu["small_config"] = er800_leg(1_000_000)
 
     north = None
     if not args.no_north_star:
@@ -299,14 +305,14 @@ def _emit(obj):
     print(json.dumps(obj), flush=True)
 
 
-def bench_er800(args) -> int:
-    """ER n=800 p=0.06 unit weights, 64 chains, 1M sweeps (linear beta 0.1->3.0,
-    P ladder 128..1): GPU run and the CPU oracle on every chain, compared bit
-    for bit per seed."""
+def er800_leg(sweeps: int, threads_per_chain: int = 0, check: bool = True) -> dict:
+    """The small configuration: ER n=800 p=0.06 unit weights, 64 chains,
+    `sweeps` sweeps (linear beta 0.1->3.0, P ladder 128..1) in one plain GPU
+    launch with autotuned geometry; with `check`, the CPU path (the oracle,
+    one chain per thread on the host's CPU share) runs every chain too, timed,
+    and the two are compared bit for bit per seed."""
     import rfmf_amd as R
     from rfmf_amd import workloads as W
-    from oracle import oracle as O
-    sweeps = args.steps * args.sweeps_per_step if args.steps != 20 else 1_000_000
     csr = W.erdos_renyi(800, 0.06, seed=0)
     C = 64
     iters = [sweeps // 8] * 8
@@ -314,15 +320,16 @@ def bench_er800(args) -> int:
     log(f"[er800] n={csr.n} m={csr.m2 // 2} B={R.max_weighted_degree(csr)} chains={C} sweeps={sum(iters)}")
     with R.Context(0) as ctx:
         gid = ctx.upload_graph(csr)
-        ms = ctx.batch_run([(gid, job)], autotune=args.threads == 0, threads=args.threads)
+        ms = ctx.batch_run([(gid, job)], autotune=threads_per_chain == 0, threads=threads_per_chain)
         geo = ctx.last_launch()
     gpu_rate = C * sum(iters) / (ms / 1e3)
-    if args.no_cpu_baseline:
+    out = {"config": "er800_p0.06_64chains", "sweeps": sum(iters), "gpu_value": gpu_rate, "gpu_kernel_ms": ms,
+           "geometry": geo, "best_cut_median": float(np.median(job.best_vals)),
+           "best_cut_max": int(job.best_vals.max())}
+    if not check:
         log(f"[er800] gpu {ms:.1f} ms ({gpu_rate:.3e} chain-steps/s, {geo}); oracle check skipped")
-        _emit({"metric": "chain-steps/sec", "config": "er800_p0.06_64chains", "sweeps": sum(iters),
-               "gpu_value": gpu_rate, "gpu_kernel_ms": ms, "geometry": geo,
-               "best_cut_median": float(np.median(job.best_vals))})
-        return 0
+        return out
+    from oracle import oracle as O
     threads, visible, model = host_cpu_share()
     end, best, cv, bv, sec = O.anneal_many(csr.offsets, csr.edge_to, csr.edge_weight, job.init, job.seeds,
                                            job.beta.run_end, job.beta.run_val, job.beta.chain_run_off,
@@ -332,12 +339,20 @@ def bench_er800(args) -> int:
     recomputed = all(R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c] for c in range(C))
     log(f"[er800] gpu {ms:.1f} ms ({gpu_rate:.3e} chain-steps/s, {geo}); cpu {sec:.1f} s on {threads} threads; "
         f"bit-exact per seed: {same}")
-    _emit({"metric": "chain-steps/sec", "config": "er800_p0.06_64chains", "sweeps": sum(iters),
-           "gpu_value": gpu_rate, "gpu_kernel_ms": ms, "geometry": geo,
-           "cpu_value": C * sum(iters) / sec, "cpu_threads": threads, "cpu_visible": visible, "cpu_model": model,
-           "cpu_seconds": sec, "bit_exact_per_seed": bool(same), "best_cut_recomputed": bool(recomputed),
-           "best_cut_median": float(np.median(job.best_vals)), "best_cut_max": int(job.best_vals.max())})
-    return 0 if same and recomputed else 1
+    out.update({"cpu_value": C * sum(iters) / sec, "cpu_threads": threads, "cpu_visible": visible,
+                "cpu_model": model, "cpu_seconds": sec, "gpu_vs_cpu": gpu_rate / (C * sum(iters) / sec),
+                "bit_exact_per_seed": bool(same), "best_cut_recomputed": bool(recomputed)})
+    return out
+
+
+def bench_er800(args) -> int:
+    """The small configuration on its own line (1M sweeps unless --steps is
+    given): GPU run and the CPU oracle on every chain, compared bit for bit
+    per seed."""
+    sweeps = args.steps * args.sweeps_per_step if args.steps != 20 else 1_000_000
+    r = er800_leg(sweeps, args.threads, check=not args.no_cpu_baseline)
+    _emit({"metric": "chain-steps/sec", **r})
+    return 0 if r.get("bit_exact_per_seed", True) and r.get("best_cut_recomputed", True) else 1
 
 
 def bench_sparse(args) -> int:

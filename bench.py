@@ -14,7 +14,7 @@ heuristic (256 at 4 chains per CU), which is the tile the live autotuner
 picks for every phase of this workload.
 
 The same line carries the batched and served modes against the plain
-one-graph-per-launch mode (`north_star`): a 256-graph mixed batch in one
+one-graph-per-launch mode (`batched_served`): a 256-graph mixed batch in one
 hipGraph-captured launch, and a Poisson stream of solve requests through the
 continuous-batching server, with latencies observed for both the server and
 a plain first-come-first-served loop.
@@ -107,7 +107,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-small-config", action="store_true",
                     help="leave the ER n=800 GPU/CPU comparison out of the cpu_baseline leg")
-    ap.add_argument("--no-north-star", action="store_true",
+    ap.add_argument("--no-batched-served", action="store_true",
                     help="headline only: skip the batched and served legs")
     # CPU sample: 4 chains per host thread (at least 64) x 64000 sweeps, about
     # 10-20 s on 16 host threads
@@ -220,9 +220,9 @@ def main() -> int:
             # both paths, partitions compared per seed.
             cpu["small_config"] = er800_leg(1_000_000)
 
-    north = None
-    if not args.no_north_star:
-        north = north_star_legs(args)
+    modes = None
+    if not args.no_batched_served:
+        modes = batched_served_legs(args)
 
     out = {
         "metric": "chain-steps/sec on 1 MI355X at equal best cut; p50/p99 solve latency",
@@ -245,7 +245,7 @@ def main() -> int:
                    "best_cut_median": float(np.median(best)), "best_cut_max": int(best.max())},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "north_star": north,
+        "batched_served": modes,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -280,7 +280,7 @@ def roofline_of(args, info, value, timed_ms, timed_sweeps, warm_sweeps, chains, 
     return out
 
 
-def north_star_legs(args) -> dict:
+def batched_served_legs(args) -> dict:
     """Batched and served modes against the plain one-graph-per-launch mode."""
     import rfmf_amd as R
     out = {}

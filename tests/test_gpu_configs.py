@@ -54,7 +54,7 @@ def test_headline_torus_at_headline_launch_shape_matches_oracle():
     sweeps = 10_000
     iters = [sweeps // 8] * 8
     job = make_job(csr, range(1, 1025), iters)
-    print(f"[config2] torus 100x200 +/-1, 1024 chains x {sweeps} sweeps, 256 threads, score table", flush=True)
+    print(f"[torus 100x200] torus 100x200 +/-1, 1024 chains x {sweeps} sweeps, 256 threads, score table", flush=True)
     with R.Context(0) as ctx:
         gid = ctx.upload_graph(csr)
         ctx.batch_run([(gid, job)], threads=256, score_table=1)
@@ -65,7 +65,7 @@ def test_headline_torus_at_headline_launch_shape_matches_oracle():
         assert R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c]
         assert R.cut_value_csr(csr, job.end_spins[c]) == job.final_vals[c]
     rows = list(range(0, 1024, 64)) + [1, 511, 1023]
-    print(f"[config2] oracle on {len(rows)} chains", flush=True)
+    print(f"[torus 100x200] oracle on {len(rows)} chains", flush=True)
     oracle_check(job, rows, "torus")
 
 
@@ -73,7 +73,7 @@ def test_headline_torus_at_headline_launch_shape_matches_oracle():
 def batch256():
     t = time.time()
     graphs = W.batch_instances(256, seed=0)
-    print(f"[config4] generated 256 graphs in {time.time() - t:.1f} s, sum n = {sum(g.n for _, g in graphs)}",
+    print(f"[256-graph batch] generated 256 graphs in {time.time() - t:.1f} s, sum n = {sum(g.n for _, g in graphs)}",
           flush=True)
     return graphs
 
@@ -83,21 +83,21 @@ def test_batch_of_256_generated_graphs_matches_oracle_and_plain_mode(batch256):
     iters = [sweeps // 8] * 8
     jobs = [make_job(g, range(64 * i + 1, 64 * i + 65), iters) for i, (_, g) in enumerate(batch256)]
     plain = [make_job(g, range(64 * i + 1, 64 * i + 65), iters) for i, (_, g) in enumerate(batch256)]
-    print(f"[config4] 256 graphs x 64 chains x {sweeps} sweeps: one hipGraph batch with autotune, then "
+    print(f"[256-graph batch] 256 graphs x 64 chains x {sweeps} sweeps: one hipGraph batch with autotune, then "
           f"one launch per graph", flush=True)
     with R.Context(0) as ctx:
         gids = [ctx.upload_graph(g) for _, g in batch256]
         ms_b = ctx.batch_run(list(zip(gids, jobs)), slice_sweeps=sweeps, use_hip_graph=True, autotune=True)
         geo = ctx.last_launch()
         ms_p = sum(ctx.batch_run([(gid, j)]) for gid, j in zip(gids, plain))
-    print(f"[config4] batched {ms_b:.1f} ms ({geo}), plain {ms_p:.1f} ms", flush=True)
+    print(f"[256-graph batch] batched {ms_b:.1f} ms ({geo}), plain {ms_p:.1f} ms", flush=True)
     for i, (a, b) in enumerate(zip(jobs, plain)):
         assert np.array_equal(a.end_spins, b.end_spins) and np.array_equal(a.best_spins, b.best_spins), i
         assert np.array_equal(a.best_vals, b.best_vals) and np.array_equal(a.final_vals, b.final_vals), i
     t = time.time()
     for i, ((fam, g), j) in enumerate(zip(batch256, jobs)):
         oracle_check(j, range(64), f"graph {i} ({fam}, n={g.n})")
-    print(f"[config4] all 16384 chains equal the oracle ({time.time() - t:.1f} s of oracle time)", flush=True)
+    print(f"[256-graph batch] all 16384 chains equal the oracle ({time.time() - t:.1f} s of oracle time)", flush=True)
 
 
 def test_poisson_request_stream_served_matches_oracle():
@@ -111,7 +111,7 @@ def test_poisson_request_stream_served_matches_oracle():
     top = sorted(fps)
     budget = top[-1] + top[-2] // 2
     assert sum(fps) > 2 * budget
-    print(f"[config5] 24 requests, chains {min(j.chains for j in jobs)}-{max(j.chains for j in jobs)}, sweeps "
+    print(f"[request stream] 24 requests, chains {min(j.chains for j in jobs)}-{max(j.chains for j in jobs)}, sweeps "
           f"{min(int(sum(j.iters)) for j in jobs)}-{max(int(sum(j.iters)) for j in jobs)}, footprints "
           f"{sum(fps)} B in all, budget {budget} B", flush=True)
     with R.Context(0) as ctx:
@@ -146,14 +146,14 @@ def test_poisson_request_stream_served_matches_oracle():
     # fit at once, so one of them is admitted only after it.
     first = td.min()
     early = ts < first
-    print(f"[config5] {int(early.sum())} requests submitted before the first completion "
+    print(f"[request stream] {int(early.sum())} requests submitted before the first completion "
           f"({sum(f for f, e in zip(fps, early) if e)} B); waits {np.round(ta - ts, 4).tolist()}", flush=True)
     if sum(f for f, e in zip(fps, early) if e) > budget:
         assert np.any(early & (ta > first))
     t = time.time()
     for i, j in enumerate(jobs):
         oracle_check(j, range(j.chains), f"request {i} (n={j.csr.n}, {j.chains} chains)")
-    print(f"[config5] every chain of every request equals the oracle ({time.time() - t:.1f} s)", flush=True)
+    print(f"[request stream] every chain of every request equals the oracle ({time.time() - t:.1f} s)", flush=True)
 
 
 @pytest.mark.parametrize("name,n,m,pm1", [("sparse_n10000_m10000_unit", 10000, 10000, False),
@@ -165,12 +165,12 @@ def test_sparse_graphs_with_4096_chains_match_oracle(name, n, m, pm1):
     csr = W.sparse_random(n, m, pm1, seed=0)
     sweeps = 400
     job = make_job(csr, range(1, 4097), [sweeps // 8] * 8)
-    print(f"[config3] {name}: n={csr.n} (after degree-zero removal), 4096 chains x {sweeps} sweeps", flush=True)
+    print(f"[sparse] {name}: n={csr.n} (after degree-zero removal), 4096 chains x {sweeps} sweeps", flush=True)
     with R.Context(0) as ctx:
         gid = ctx.upload_graph(csr)
         ctx.batch_run([(gid, job)], autotune=True)
         geo = ctx.last_launch()
-    print(f"[config3] {name}: {geo}", flush=True)
+    print(f"[sparse] {name}: {geo}", flush=True)
     for c in range(4096):
         assert R.cut_value_csr(csr, job.best_spins[c]) == job.best_vals[c]
         assert R.cut_value_csr(csr, job.end_spins[c]) == job.final_vals[c]

@@ -136,7 +136,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the annealer shards by chains and graphs with no cross-GPU "
-              f"exchange, and this build runs on one MI355X only (asked for --gpus {args.gpus}).",
+              f"exchange, and this library runs on one MI355X only (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         return 2
 

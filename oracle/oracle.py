@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
 
 Test infrastructure only: imported by tests/, by the smoke check and by the
-CPU-baseline leg of bench.py, always as the checker or the baseline, never as
+CPU-baseline leg of bench.py, always as the parity reference or the baseline, never as
 the thing measured or shipped.
 """
 from __future__ import annotations

@@ -2,7 +2,7 @@
  * rfmf_oracle.h -- CPU restatement of the RFMF multi-chain rejection-free
  * multiple-flip annealer, written as TEST INFRASTRUCTURE.
  *
- * This library is the parity checker for the HIP path and the timed CPU
+ * This library is the parity oracle for the HIP path and the timed CPU
  * baseline.  Only the test suite, the smoke check and the CPU-baseline leg of
  * the benchmark load it; the product library never links it, never calls it
  * and has no CPU fallback through it.

@@ -1,4 +1,4 @@
-"""The reference's shell tests, run against this build's CLI (rfmf_main).
+"""The reference's shell tests, run against the rfmf_main CLI.
 
 * Cut consistency (tests/test_cut_consistency.sh): for every committed Gset
   instance, each chain's incremental best/final cut (stderr

@@ -141,7 +141,7 @@ def test_replay_every_committed_chain_of_a_run(run_name):
 def test_cli_reproduces_the_committed_g62_log():
     """The reference's committed command for G62 (results/V100-SXM2-10M/
     reproduce.sh: 80 seeds and t0s, ladder 128..1, 10^7 sweeps per phase) run
-    through this build's CLI: every stdout line equals the committed G62.out
+    through the rfmf_main CLI: every stdout line equals the committed G62.out
     line (rebuilt from the fixtures: the device lines, both partitions of all
     80 chains, the summary) with time-sec masked, and the stderr banner and
     per-chain kernel values equal the committed G62.err lines."""

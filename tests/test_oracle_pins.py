@@ -1,5 +1,5 @@
 """Pins the CPU oracle (oracle/rfmf_oracle.c) before it is trusted as the
-parity checker: published known-answer vectors, closed forms, brute force on
+parity reference: published known-answer vectors, closed forms, brute force on
 tiny inputs, vectors written by the reference's own host code
 (tests/golden/ref_vectors.json) and the reference's committed benchmark
 partitions (tests/golden/committed_chains.json)."""

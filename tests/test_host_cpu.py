@@ -218,6 +218,22 @@ def test_anneal_passthrough_without_phases():
     assert [int(x) for x in r.kernel_best_vals] == [R.cut_value_csr(csr, init[c]) for c in range(2)]
 
 
+def test_anneal_passthrough_on_an_empty_graph(tmp_path):
+    """A graph left with no vertices (every edge weight zero, so every vertex
+    has degree zero and is removed) anneals to a pass-through, as
+    src/anneal_kernels.cu:856-867 returns it: no device work, zero time, the
+    empty partitions and cuts of 0."""
+    p = tmp_path / "zero_weights"
+    p.write_text("3 2\n1 2 0\n2 3 0\n")
+    csr, st = R.read_instance_from_file(str(p))
+    assert (st.num_vertices, st.zero_dropped, csr.n, csr.m2) == (3, 2, 0, 0)
+    init = np.zeros((2, 0), dtype=np.int8)
+    r = R.anneal_gpu(csr, R.max_weighted_degree(csr), init, [1.0, 2.0], [8, 1], [4, 4], [5, 6])
+    assert r.end_spins.shape == (2, 0) and r.best_spins.shape == (2, 0)
+    assert r.kernel_ms == 0.0
+    assert [int(x) for x in r.kernel_best_vals] == [0, 0] and [int(x) for x in r.kernel_final_vals] == [0, 0]
+
+
 def test_context_without_device_reports_no_device():
     import os
     if os.path.exists("/dev/kfd"):

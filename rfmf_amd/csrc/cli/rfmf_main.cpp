@@ -19,6 +19,7 @@
 
 #include "../../../include/rfmf.h"
 #include "../host/anneal_gpu.hpp"
+#include "../host/tuning.hpp"
 
 namespace {
 
@@ -169,7 +170,7 @@ int main(int argc, char** argv) {
     gpu3::LaunchConfig cfg;
     cfg.device = a.device;
     const double meanDeg = n > 0 ? static_cast<double>(view.m2) / static_cast<double>(n) : 0.0;
-    cfg.corr = meanDeg >= 8.0 ? gpu3::CorrMode::Warpwalk : gpu3::CorrMode::Walk;
+    cfg.corr = meanDeg >= gpu3::tuning::corrWarpwalkMinDeg ? gpu3::CorrMode::Warpwalk : gpu3::CorrMode::Walk;
     cfg.sWidth = B <= 2147483647LL ? 32 : 64;
     int64_t total = 0;
     if (rfmf_total_steps(a.iters.data(), static_cast<int64_t>(a.iters.size()), &total) != RFMF_OK) {

@@ -16,6 +16,7 @@
 #include <unordered_set>
 
 #include "../host/max_cut.hpp"
+#include "../host/tuning.hpp"
 
 namespace rfmf {
 
@@ -289,9 +290,11 @@ int heuristicThreads(const hipDeviceProp_t& props, int bits, int mode, int64_t m
     // fewer, 8 waves per chain cut the per-sweep latency (16 waves add more
     // per-wave work than they hide), and 16 when the state is in HBM, whose
     // loads want more waves in flight.
+    namespace T = gpu3::tuning;
     const double perCu = static_cast<double>(chains) / std::max(1, props.multiProcessorCount);
-    int t = hbm ? 1024 : 512;
-    if (perCu >= 2.0) t = (hbm || maxN <= 2048) ? 128 : 256;
+    int t = hbm ? T::tpbStateInHbm : maxN <= T::tpbWalkSmallMaxN ? T::tpbWalkSmall : T::tpbWalkLarge;
+    if (perCu >= T::coResidentMinChainsPerCu)
+        t = (hbm || maxN <= T::blocked8CoResBigMinN) ? T::tpbCoResidentBlocked8 : T::tpbCoResidentBlocked8BigN;
     while (t > 128 && !fitsLds(props, maxN, bits, mode, maxP, t, hbm)) t /= 2;
     return t;
 }
